@@ -1,0 +1,179 @@
+"""Benchmark of the lattice step on one MI355X.
+
+Headline workload: D3Q19 TRT body-force channel through a Bernoulli(0.3)
+porous solid, 256^3 nodes, periodic in x, no-slip y/z walls, force 1e-6
+along x (magic parameter 3/16, tau 0.8), populations in FP32. One "step" is
+one lattice time step of the whole 256^3 lattice. MLUPS = N * steps / t / 1e6.
+
+Prints one JSON line. Besides the headline (FP32), the same workload is
+timed in f64 (the parity path) and in the shifted FP16 storage tier when it
+is built, and reported under "extra". The roofline block uses algorithmic
+bytes 2*Q*s per node update; the CPU baseline times the oracle (the
+line-by-line f64 restatement of the reference solver) on the host cores for
+a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & % HBM peak, D3Q19 256³; served cases/s + p99 latency vs plain f64"
+
+
+class Hip:
+    """Minimal HIP event timing through libamdhip64 (stream of the case)."""
+
+    def __init__(self):
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.lib = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        L = self.lib
+        L.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        L.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        L.hipEventSynchronize.argtypes = [C.c_void_p]
+        L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        L.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def event(self):
+        e = C.c_void_p()
+        assert self.lib.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.lib.hipEventRecord(e, stream) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.lib.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.lib.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return ms.value
+
+
+def time_case(spec, steps, warmup, hip):
+    from lbflow_amd.engine import Case, load_library
+    lib = load_library()
+    lib.lbfx_case_stream.restype = C.c_void_p
+    lib.lbfx_case_stream.argtypes = [C.c_void_p]
+    case = Case(spec)
+    stream = C.c_void_p(lib.lbfx_case_stream(case.h))
+    case.initialize_nodes()
+    case.main_steps(warmup)
+    case.synchronize()
+    e0, e1 = hip.event(), hip.event()
+    t0 = time.perf_counter()
+    hip.record(e0, stream)
+    case.main_steps(steps)
+    hip.record(e1, stream)
+    case.synchronize()
+    wall = time.perf_counter() - t0
+    ms = hip.elapsed_ms(e0, e1)
+    from lbflow_amd import abi
+    nonfinite = case.reduce(abi.REDUCE_NONFINITE)
+    mass = case.reduce(abi.REDUCE_TOTAL_MASS)
+    case.close()
+    return ms, wall, nonfinite, mass
+
+
+def cpu_baseline(n=128, steps=150, threads=None):
+    """Times the oracle on a bounded sample (n^3 porous channel, same physics)."""
+    from lbflow_amd import configs
+    from oracle import oracle
+    oracle.load()
+    lib = oracle.load()
+    if threads:
+        lib.lbo_set_threads(int(threads))
+    cores = lib.lbo_max_threads()
+    sp = configs.porous_channel(n=n, precision=0)
+    o = oracle.OracleCase(sp)
+    o.initialize_nodes()
+    o.main_steps(1)
+    t0 = time.perf_counter()
+    o.main_steps(steps)
+    dt = time.perf_counter() - t0
+    mlups = sp.num_nodes * steps / dt / 1e6
+    o.close()
+    return {"value": round(mlups, 3), "unit": "MLUPS", "cores": int(cores), "kind": "port",
+            "sample": f"oracle (C f64 restatement of the reference step, OpenMP) on the same "
+                      f"D3Q19 TRT porous channel at {n}^3, {steps} timed steps after 1 warm-up "
+                      f"({dt:.1f} s)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of variants (f32,f64,f16s,...)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              f"(every case fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from lbflow_amd import abi, configs
+    hip = Hip()
+    n = args.n
+    N = n ** 3
+    variants = [("f32", abi.F32, 4), ("f64", abi.F64, 8)]
+    if args.only:
+        keep = set(args.only.split(","))
+        variants = [v for v in variants if v[0] in keep]
+    results = {}
+    for name, prec, sbytes in variants:
+        sp = configs.porous_channel(n=n, precision=prec)
+        print(f"[bench] {name}: D3Q19 TRT porous {n}^3, warmup {args.warmup}, steps {args.steps}",
+              file=sys.stderr, flush=True)
+        ms, wall, nonfinite, mass = time_case(sp, args.steps, args.warmup, hip)
+        mlups = N * args.steps / (ms / 1e3) / 1e6
+        gbs = mlups * 1e6 * 2 * 19 * sbytes / 1e9
+        results[name] = dict(mlups=round(mlups, 1), ms_per_step=round(ms / args.steps, 4),
+                             achieved_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
+                             wall_s=round(wall, 3), nonfinite=nonfinite, mass=mass,
+                             bytes_per_site=2 * 19 * sbytes)
+        print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)",
+              file=sys.stderr, flush=True)
+    head = results.get("f32") or next(iter(results.values()))
+    head_name = "f32" if "f32" in results else next(iter(results))
+    cpu = None if args.no_cpu else cpu_baseline()
+    bytes_per_launch = N * head["bytes_per_site"]
+    line = {
+        "metric": METRIC,
+        "value": head["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": head_name,
+        "data": "synthetic (seeded Bernoulli(0.3) solid mask, rest-state start)",
+        "config": {"workload": f"D3Q19 TRT porous channel {n}^3, Bernoulli(0.3) solids seed 2, "
+                               f"periodic x, no-slip y/z, force 1e-6 x, magic 3/16, tau 0.8, "
+                               f"{head_name} populations, two-array fused pull-stream kernel",
+                   "lattice": "D3Q19", "nodes": N, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["hbm_frac"], "traffic": None,
+                     "algorithmic_bytes_per_launch": bytes_per_launch},
+        "cpu_baseline": cpu,
+        "extra": results,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+"""Case builders for the benchmark configurations (sizes are parameters, so
+the same builders give the small parity cases and the full-size runs)."""
+from __future__ import annotations
+
+import numpy as np
+
+from . import abi
+from .case import Collision, FaceBC, ScalarFaceBC, ScalarSpec, Spec
+from .geometry import bernoulli_mask, discs_mask, sphere_mask
+
+
+def cavity(n=128, tau=0.6, lid=0.1, precision=abi.F64, op=None) -> Spec:
+    """D2Q9 lid-driven cavity: the reference's default parameters
+    (src/momentum.rs:39-69) with n and tau changed — W/E/S no-slip, N a
+    moving bounce-back wall (rho 1, u = (lid, 0)), rest-state start."""
+    return Spec(
+        velocity_set=abi.D2Q9, n=(n, n), collision=op or Collision.bgk(tau),
+        faces=[FaceBC.no_slip(), FaceBC.no_slip(), FaceBC.no_slip(),
+               FaceBC.bounce_back(1.0, (lid, 0.0))],
+        initial_density=1.0, initial_velocity=(0.0, 0.0), precision=precision)
+
+
+def porous_channel(n=256, precision=abi.F64, p_solid=0.3, seed=2, tau=0.8, magic=3.0 / 16.0,
+                   force=1e-6, op=None, velocity_set=abi.D3Q19, streaming=abi.STREAM_AUTO) -> Spec:
+    """D3Q19 TRT body-force channel through a Bernoulli(p_solid) porous solid:
+    periodic in x, no-slip walls on y/z, force along x."""
+    shape = (n, n, n) if np.ndim(n) == 0 else tuple(n)
+    mask = bernoulli_mask(shape, p_solid, seed) if p_solid > 0 else None
+    return Spec(
+        velocity_set=velocity_set, n=shape, collision=op or Collision.trt_magic(tau, magic),
+        faces=[FaceBC.periodic(), FaceBC.periodic(), FaceBC.no_slip(), FaceBC.no_slip(),
+               FaceBC.no_slip(), FaceBC.no_slip()],
+        node_type=mask, force=(force, 0.0, 0.0), initial_density=1.0,
+        initial_velocity=(0.0, 0.0, 0.0), precision=precision, streaming=streaming)
+
+
+def periodic_box(n=256, precision=abi.F64, tau=0.8, magic=3.0 / 16.0, velocity_set=abi.D3Q19,
+                 streaming=abi.STREAM_AUTO, op=None) -> Spec:
+    """All-fluid fully periodic box (the clean roofline point), TRT."""
+    shape = (n, n, n) if np.ndim(n) == 0 else tuple(n)
+    return Spec(
+        velocity_set=velocity_set, n=shape, collision=op or Collision.trt_magic(tau, magic),
+        faces=[FaceBC.periodic()] * 6, initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0),
+        precision=precision, streaming=streaming)
+
+
+def coupled_discs(nx=1024, ny=512, count=200, rmin=4.0, rmax=12.0, seed=4, tau=0.6, tau_g=0.6,
+                  u_in=0.05, k_a=1e-3, q_max=1.0, precision=abi.F32,
+                  zeroth_unknown_only=False) -> Spec:
+    """D2Q9 BGK flow + coupled D2Q9 passive scalar with Langmuir adsorption on
+    the walls of random disc obstacles: Zou-He velocity inlet (W) and
+    pressure outlet (E), no-slip S/N; scalar c = 1 at the inlet
+    (anti-bounce-back), zeroth-order outflow, no-flux S/N, inner bounce-back."""
+    mask = discs_mask(nx, ny, count, rmin, rmax, seed)
+    scalar = ScalarSpec(
+        collision=Collision.bgk(tau_g),
+        faces=[ScalarFaceBC(abi.SBC_ANTI_BOUNCE_BACK, 1.0),
+               ScalarFaceBC(abi.SBC_ZEROTH_ORDER_NO_FLUX, unknown_populations_only=zeroth_unknown_only),
+               ScalarFaceBC(abi.SBC_BB_NO_FLUX), ScalarFaceBC(abi.SBC_BB_NO_FLUX)],
+        inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.0,
+        adsorption=dict(adsorption_rate=k_a, desorption_rate=0.0, q_max=q_max, initial_q_ads=0.0))
+    return Spec(
+        velocity_set=abi.D2Q9, n=(nx, ny), collision=Collision.bgk(tau),
+        faces=[FaceBC.zou_he(None, (u_in, 0.0)), FaceBC.zou_he(1.0, (None, 0.0)),
+               FaceBC.no_slip(), FaceBC.no_slip()],
+        node_type=mask, initial_density=1.0, initial_velocity=(0.0, 0.0), scalars=[scalar],
+        precision=precision)
+
+
+def sphere_channel(nx=384, ny=128, nz=128, radius=16, u_in=0.05, tau=0.6, precision=abi.F64,
+                   velocity_set=abi.D3Q27, op=None) -> Spec:
+    """Flow past a sphere (bounce-back), periodic lateral faces, velocity given
+    by a uniform body force so every face stays reference-constructible."""
+    mask = sphere_mask(nx, ny, nz, (nx // 4, ny // 2, nz // 2), radius)
+    return Spec(
+        velocity_set=velocity_set, n=(nx, ny, nz), collision=op or Collision.bgk(tau),
+        faces=[FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 4, node_type=mask,
+        initial_density=1.0, initial_velocity=(u_in, 0.0, 0.0), precision=precision)

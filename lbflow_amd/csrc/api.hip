@@ -1,0 +1,816 @@
+// api.hip — host implementation of the C ABI declared in include/lbflow_hip.h.
+//
+// A case handle owns its device buffers and one HIP stream (default
+// priority). Geometry classes are built once on the host, bit-exact with the
+// reference's construction (src/momentum/lattice.rs:127-191), and uploaded as
+// one 32-bit class word per node. The time loop runs on the device: every
+// step is a fixed sequence of launches whose last one advances the device
+// step counter, so lbf_main_steps and lbf_run only enqueue work.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lbflow_hip.h"
+#include "dispatch.hpp"
+#include "engine.hpp"
+
+using namespace lbf;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(LBF_EDEVICE, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                  __FILE__, __LINE__);                                                 \
+  } while (0)
+
+struct VsInfo {
+  int d, q;
+  bool mrt;
+  int c[27][3];
+};
+
+bool vs_info(int id, VsInfo* v) {
+  static const int c9[9][3] = {{0, 0, 0}, {1, 0, 0},  {0, 1, 0},   {-1, 0, 0}, {0, -1, 0},
+                               {1, 1, 0}, {-1, 1, 0}, {-1, -1, 0}, {1, -1, 0}};
+  static const int c15[15][3] = {{0, 0, 0},   {1, 0, 0},  {-1, 0, 0},  {0, 1, 0},  {0, -1, 0},
+                                 {0, 0, 1},   {0, 0, -1}, {1, 1, 1},   {-1, -1, -1}, {1, 1, -1},
+                                 {-1, -1, 1}, {1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},  {1, -1, -1}};
+  static const int c27[27][3] = {
+      {0, 0, 0},   {1, 0, 0},   {-1, 0, 0}, {0, 1, 0},   {0, -1, 0},  {0, 0, 1},   {0, 0, -1},
+      {1, 1, 0},   {-1, -1, 0}, {1, 0, 1},  {-1, 0, -1}, {0, 1, 1},   {0, -1, -1}, {1, -1, 0},
+      {-1, 1, 0},  {1, 0, -1},  {-1, 0, 1}, {0, 1, -1},  {0, -1, 1},  {1, 1, 1},   {-1, -1, -1},
+      {1, 1, -1},  {-1, -1, 1}, {1, -1, 1}, {-1, 1, -1}, {-1, 1, 1},  {1, -1, -1}};
+  switch (id) {
+    case LBF_D2Q9:
+      v->d = 2, v->q = 9, v->mrt = true;
+      memcpy(v->c, c9, sizeof c9);
+      return true;
+    case LBF_D3Q15:
+      v->d = 3, v->q = 15, v->mrt = true;
+      memcpy(v->c, c15, sizeof c15);
+      return true;
+    case LBF_D3Q19:  // first 19 directions of the D3Q27 ordering
+      v->d = 3, v->q = 19, v->mrt = true;
+      memcpy(v->c, c27, sizeof(int) * 3 * 19);
+      return true;
+    case LBF_D3Q27:
+      v->d = 3, v->q = 27, v->mrt = true;  // own basis (no reference MRT)
+      memcpy(v->c, c27, sizeof c27);
+      return true;
+  }
+  return false;
+}
+
+bool get_launchers(int vs, int prec, Launchers* L) {
+  switch (vs) {
+    case LBF_D2Q9: return launchers_d2q9(prec, L);
+    case LBF_D3Q15: return launchers_d3q15(prec, L);
+    case LBF_D3Q19: return launchers_d3q19(prec, L);
+    case LBF_D3Q27: return launchers_d3q27(prec, L);
+  }
+  return false;
+}
+
+size_t storage_bytes(int prec) {
+  switch (prec) {
+    case LBF_F64: return 8;
+    case LBF_F32:
+    case LBF_F32_SHIFTED: return 4;
+    default: return 2;
+  }
+}
+
+}  // namespace
+
+struct ScalarDev {
+  void* pop[2] = {nullptr, nullptr};
+  void* c[2] = {nullptr, nullptr};
+  double* q_ads = nullptr;
+  double* src = nullptr;
+  lbf_scalar_desc desc{};
+};
+
+struct lbf_case {
+  lbf_case_desc desc{};
+  VsInfo v{};
+  int d = 0, q = 0, nx = 1, ny = 1, nz = 1;
+  long long N = 0, ld = 0;
+  int prec = LBF_F64;
+  size_t sb = 8, tb = 8;  // storage / compute bytes
+  int device = 0;
+  int grid = 1;
+  hipStream_t stream = nullptr;
+  Launchers L{};
+  // device memory
+  void* pop[2] = {nullptr, nullptr};
+  uint32_t* cls = nullptr;
+  void* rho[2] = {nullptr, nullptr};
+  void* u[2] = {nullptr, nullptr};
+  double* force = nullptr;
+  Ctrl* ctrl = nullptr;
+  double* partials = nullptr;
+  double* readback = nullptr;
+  double* red_out = nullptr;
+  ScalarDev sc[LBF_MAX_SCALARS];
+  int ns = 0;
+  bool frozen = false;
+  bool needs_moments = false;  // some condition reads stored moments every step
+  uint32_t face_active = 0;
+  // host copies of the node classes
+  std::vector<uint8_t> h_type;
+  std::vector<uint32_t> h_mask;
+  std::vector<uint8_t> h_faces;
+  KArgs base{};
+};
+
+namespace {
+
+void free_case(lbf_case* c) {
+  if (!c) return;
+  auto F = [](void* p) {
+    if (p) (void)hipFree(p);
+  };
+  F(c->pop[0]), F(c->pop[1]), F(c->cls), F(c->rho[0]), F(c->rho[1]), F(c->u[0]), F(c->u[1]),
+      F(c->force), F(c->ctrl), F(c->partials), F(c->readback), F(c->red_out);
+  for (int s = 0; s < c->ns; s++) {
+    ScalarDev& S = c->sc[s];
+    F(S.pop[0]), F(S.pop[1]), F(S.c[0]), F(S.c[1]), F(S.q_ads), F(S.src);
+  }
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int check_collision(const VsInfo& v, const lbf_collision& col, const char* what) {
+  if (col.op == LBF_BGK) {
+    if (!(col.tau > 0)) return fail(LBF_EINVAL, "%s: BGK tau must be > 0", what);
+  } else if (col.op == LBF_TRT) {
+  } else if (col.op == LBF_MRT) {
+    if (!v.mrt) return fail(LBF_EUNSUPPORTED, "%s: MRT not available for this velocity set", what);
+  } else {
+    return fail(LBF_EINVAL, "%s: unknown collision operator %d", what, col.op);
+  }
+  return 0;
+}
+
+// Zou-He combinations accepted by src/velocity_set/d2q9.rs:173-242
+bool zou_he_ok(int face, unsigned known) {
+  const bool hd = known & LBF_KNOWN_DENSITY, hx = known & LBF_KNOWN_UX, hy = known & LBF_KNOWN_UY;
+  if (face == LBF_WEST || face == LBF_EAST) return (!hd && hx && hy) || (hd && !hx && hy);
+  if (face == LBF_SOUTH || face == LBF_NORTH) return (!hd && hx && hy) || (hd && hx && !hy);
+  return false;
+}
+
+void fill_collision(KArgs& a, const lbf_collision& col, int q) {
+  a.op = col.op;
+  a.omega = 1.0 / col.tau;        // DELTA_T / tau, src/kernel.rs:35
+  a.omega_p = 1.0 - a.omega;      // src/kernel.rs:36
+  a.coeff_b = 1.0 - 0.5 / col.tau;  // src/kernel.rs:131 and :163
+  a.wp = col.omega_plus;
+  a.wm = col.omega_minus;
+  for (int k = 0; k < 27; k++) a.s[k] = k < q ? col.relaxation[k] : 0.0;
+}
+
+template <class T>
+void to_compute(const double* in, size_t n, std::vector<char>& out) {
+  out.resize(n * sizeof(T));
+  T* o = reinterpret_cast<T*>(out.data());
+  for (size_t i = 0; i < n; i++) o[i] = static_cast<T>(in[i]);
+}
+
+int upload_compute(lbf_case* c, void* dst, const double* src, size_t n) {
+  std::vector<char> buf;
+  if (c->tb == 8)
+    to_compute<double>(src, n, buf);
+  else
+    to_compute<float>(src, n, buf);
+  HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+int download_compute(lbf_case* c, const void* src, double* out, size_t n) {
+  std::vector<char> buf(n * c->tb);
+  HIPCHK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+  if (c->tb == 8)
+    memcpy(out, buf.data(), n * 8);
+  else
+    for (size_t i = 0; i < n; i++) out[i] = reinterpret_cast<const float*>(buf.data())[i];
+  return 0;
+}
+
+int sync_and_check(lbf_case* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+int read_ctrl(lbf_case* c, Ctrl* h) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int validate(const lbf_case_desc* d, VsInfo* v) {
+  if (!d) return fail(LBF_EINVAL, "descriptor is NULL");
+  if (d->abi_version != LBF_ABI_VERSION)
+    return fail(LBF_EINVAL, "ABI version %u, library expects %u", d->abi_version, LBF_ABI_VERSION);
+  if (!vs_info(d->velocity_set, v))
+    return fail(LBF_EUNSUPPORTED, "velocity set %d is not constructible", d->velocity_set);
+  for (int a = 0; a < v->d; a++)
+    if (d->n[a] < 1) return fail(LBF_EINVAL, "lattice dimension %d must be >= 1", a);
+  int rc = check_collision(*v, d->collision, "momentum");
+  if (rc) return rc;
+  const int nf = 2 * v->d;
+  for (int f = 0; f < nf; f++) {
+    const int k = d->faces[f].kind;
+    if (k < LBF_BC_NO_SLIP || k > LBF_BC_PERIODIC)
+      return fail(LBF_EINVAL, "face %d has no boundary condition", f);
+    if (k == LBF_BC_ZOU_HE) {
+      if (d->velocity_set != LBF_D2Q9)
+        return fail(LBF_EUNSUPPORTED, "Zou & He BC computation function not defined.");
+      if (!zou_he_ok(f, d->faces[f].known))
+        return fail(LBF_EINVAL, "Error: Wrong input of Zou & He (1997) boundary conditions.");
+    }
+  }
+  if (d->precision < LBF_F64 || d->precision > LBF_BF16_SHIFTED)
+    return fail(LBF_EINVAL, "unknown precision %d", d->precision);
+  if (d->n_scalars > LBF_MAX_SCALARS)
+    return fail(LBF_EUNSUPPORTED, "at most %d scalars", LBF_MAX_SCALARS);
+  if (d->n_scalars && !d->scalars) return fail(LBF_EINVAL, "scalars pointer is NULL");
+  if (d->force_kind == LBF_SOURCE_FIELD && !d->force_field)
+    return fail(LBF_EINVAL, "force field pointer is NULL");
+  for (uint32_t s = 0; s < d->n_scalars; s++) {
+    const lbf_scalar_desc& S = d->scalars[s];
+    if (S.velocity_set != d->velocity_set)
+      return fail(LBF_EUNSUPPORTED, "scalar %u: velocity set must equal the momentum one", s);
+    rc = check_collision(*v, S.collision, "scalar");
+    if (rc) return rc;
+    for (int f = 0; f < nf; f++) {
+      const int k = S.faces[f].kind;
+      if (k < LBF_SBC_ANTI_BOUNCE_BACK || k > LBF_SBC_PERIODIC)
+        return fail(LBF_EINVAL, "scalar %u face %d has no boundary condition", s, f);
+    }
+    if (S.source_kind == LBF_SOURCE_FIELD && !S.source_field)
+      return fail(LBF_EINVAL, "scalar %u source field pointer is NULL", s);
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lbf_last_error(void) { return g_err.c_str(); }
+
+int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
+  VsInfo v;
+  int rc = validate(d, &v);
+  if (rc) return rc;
+  const long long N = (long long)d->n[0] * d->n[1] * (v.d == 3 ? d->n[2] : 1);
+  const long long ld = (N + 63) / 64 * 64;
+  const size_t sb = storage_bytes(d->precision), tb = d->precision == LBF_F64 ? 8 : 4;
+  uint64_t b = 2ull * v.q * ld * sb + 4ull * ld + 2ull * (1 + v.d) * ld * tb;
+  if (d->force_kind == LBF_SOURCE_FIELD) b += 8ull * v.d * ld;
+  for (uint32_t s = 0; s < d->n_scalars; s++) {
+    b += 2ull * v.q * ld * tb + 2ull * ld * tb;
+    if (d->scalars[s].has_adsorption) b += 8ull * ld;
+    if (d->scalars[s].source_kind == LBF_SOURCE_FIELD) b += 8ull * ld;
+  }
+  *bytes = b + sizeof(Ctrl) + 2048ull * kMaxCols * 8;
+  return 0;
+}
+
+int lbf_create(const lbf_case_desc* d, lbf_case** out) {
+  if (!out) return fail(LBF_EINVAL, "out is NULL");
+  *out = nullptr;
+  VsInfo v;
+  int rc = validate(d, &v);
+  if (rc) return rc;
+  lbf_case* c = new lbf_case();
+  c->desc = *d;
+  c->v = v;
+  c->d = v.d, c->q = v.q;
+  c->nx = (int)d->n[0], c->ny = (int)d->n[1], c->nz = v.d == 3 ? (int)d->n[2] : 1;
+  c->N = (long long)c->nx * c->ny * c->nz;
+  c->ld = (c->N + 63) / 64 * 64;
+  c->prec = d->precision;
+  c->sb = storage_bytes(c->prec);
+  c->tb = c->prec == LBF_F64 ? 8 : 4;
+  c->frozen = (d->flags & LBF_CASE_FREEZE_MOMENTUM) != 0;
+  c->ns = (int)d->n_scalars;
+  if (!get_launchers(d->velocity_set, c->prec, &c->L)) {
+    delete c;
+    return fail(LBF_EUNSUPPORTED, "precision %d is not built for velocity set %d", d->precision,
+                d->velocity_set);
+  }
+  const int nf = 2 * c->d;
+  for (int f = 0; f < nf; f++)
+    if (d->faces[f].kind != LBF_BC_PERIODIC) c->face_active |= 1u << f;
+  for (int f = 0; f < nf; f++)
+    if (d->faces[f].kind == LBF_BC_ANTI_BOUNCE_BACK) c->needs_moments = true;
+  if (c->ns) c->needs_moments = true;
+
+  // ---- node classes (lattice.rs:127-191): periodic neighbours, solid links
+  const long long N = c->N;
+  const int q = c->q;
+  c->h_type.assign(N, 0);
+  if (d->node_type)
+    for (long long i = 0; i < N; i++) {
+      if (d->node_type[i] > 1) {
+        free_case(c);
+        return fail(LBF_EINVAL, "Invalid node type at node %lld", i);
+      }
+      c->h_type[i] = d->node_type[i];
+    }
+  c->h_mask.assign(N, 0);
+  c->h_faces.assign(N, 0);
+  std::vector<uint32_t> cls(c->ld, kSolidBit);
+  const long long n3[3] = {c->nx, c->ny, c->nz};
+  for (long long i = 0; i < N; i++) {
+    const long long x = i % c->nx, y = (i / c->nx) % c->ny, z = i / ((long long)c->nx * c->ny);
+    const long long idx[3] = {x, y, z};
+    uint32_t m = 0;
+    if (!c->h_type[i]) {
+      for (int k = 0; k < q; k++) {
+        long long nb[3] = {x, y, z};
+        for (int a = 0; a < c->d; a++) nb[a] = ((idx[a] + v.c[k][a]) % n3[a] + n3[a]) % n3[a];
+        const long long j = nb[0] + n3[0] * (nb[1] + n3[1] * nb[2]);
+        if (c->h_type[j]) m |= 1u << k;
+      }
+      uint8_t fb = 0;
+      for (int f = 0; f < nf; f++) {
+        const int a = f / 2;
+        const long long lim = (f % 2) ? n3[a] - 1 : 0;
+        if (idx[a] == lim) fb |= (uint8_t)(1u << f);
+      }
+      c->h_faces[i] = fb;
+      c->h_mask[i] = m;
+      cls[i] = m;
+    } else {
+      cls[i] = kSolidBit;
+    }
+  }
+
+  // ---- device memory
+  auto cleanup = [&](int code) {
+    free_case(c);
+    return code;
+  };
+  if (hipGetDevice(&c->device) != hipSuccess) return cleanup(fail(LBF_EDEVICE, "no HIP device"));
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c->device) != hipSuccess)
+    return cleanup(fail(LBF_EDEVICE, "hipGetDeviceProperties failed"));
+  const long long blocks = (N + kBlock - 1) / kBlock;
+  c->grid = (int)std::max(1LL, std::min<long long>(blocks, (long long)prop.multiProcessorCount * 8));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
+  auto alloc = [&](void** p, size_t bytes) -> int {
+    if (hipMalloc(p, bytes) != hipSuccess)
+      return fail(LBF_ENOMEM, "hipMalloc(%zu) failed", bytes);
+    if (hipMemset(*p, 0, bytes) != hipSuccess) return fail(LBF_EDEVICE, "hipMemset failed");
+    return 0;
+  };
+  const size_t pop_bytes = (size_t)q * c->ld * c->sb;
+  for (int b = 0; b < 2; b++) {
+    if ((rc = alloc(&c->pop[b], pop_bytes))) return cleanup(rc);
+    if ((rc = alloc(&c->rho[b], c->ld * c->tb))) return cleanup(rc);
+    if ((rc = alloc(&c->u[b], (size_t)c->d * c->ld * c->tb))) return cleanup(rc);
+  }
+  if ((rc = alloc((void**)&c->cls, c->ld * 4))) return cleanup(rc);
+  if (hipMemcpy(c->cls, cls.data(), c->ld * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(LBF_EDEVICE, "upload of node classes failed"));
+  if ((rc = alloc((void**)&c->ctrl, sizeof(Ctrl)))) return cleanup(rc);
+  {
+    Ctrl h{};
+    h.max_iter = ~0ull;
+    if (hipMemcpy(c->ctrl, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(LBF_EDEVICE, "upload of control block failed"));
+  }
+  if ((rc = alloc((void**)&c->partials, (size_t)c->grid * kMaxCols * 8))) return cleanup(rc);
+  if ((rc = alloc((void**)&c->red_out, 64))) return cleanup(rc);
+
+  // initial moments into both buffers (solid nodes keep them forever)
+  {
+    std::vector<double> rho(c->ld, 0.0), u((size_t)c->d * c->ld, 0.0);
+    for (long long i = 0; i < N; i++) {
+      rho[i] = d->initial_density ? d->initial_density[i] : d->initial_density_uniform;
+      for (int a = 0; a < c->d; a++)
+        u[a * c->ld + i] =
+            d->initial_velocity ? d->initial_velocity[i * c->d + a] : d->initial_velocity_uniform[a];
+    }
+    for (int b = 0; b < 2; b++) {
+      if ((rc = upload_compute(c, c->rho[b], rho.data(), c->ld))) return cleanup(rc);
+      if ((rc = upload_compute(c, c->u[b], u.data(), (size_t)c->d * c->ld))) return cleanup(rc);
+    }
+  }
+  if (d->force_kind == LBF_SOURCE_FIELD) {
+    std::vector<double> F((size_t)c->d * c->ld, 0.0);
+    for (long long i = 0; i < N; i++)
+      for (int a = 0; a < c->d; a++) F[a * c->ld + i] = d->force_field[i * c->d + a];
+    if ((rc = alloc((void**)&c->force, F.size() * 8))) return cleanup(rc);
+    if (hipMemcpy(c->force, F.data(), F.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return cleanup(fail(LBF_EDEVICE, "upload of force field failed"));
+  }
+  for (int s = 0; s < c->ns; s++) {
+    ScalarDev& S = c->sc[s];
+    S.desc = d->scalars[s];
+    S.desc.initial_value = nullptr;
+    S.desc.source_field = nullptr;
+    const lbf_scalar_desc& sd = d->scalars[s];
+    for (int b = 0; b < 2; b++) {
+      if ((rc = alloc(&S.pop[b], (size_t)q * c->ld * c->tb))) return cleanup(rc);
+      if ((rc = alloc(&S.c[b], c->ld * c->tb))) return cleanup(rc);
+    }
+    std::vector<double> cv(c->ld, 0.0);
+    for (long long i = 0; i < N; i++)
+      cv[i] = sd.initial_value ? sd.initial_value[i] : sd.initial_value_uniform;
+    for (int b = 0; b < 2; b++)
+      if ((rc = upload_compute(c, S.c[b], cv.data(), c->ld))) return cleanup(rc);
+    if (sd.has_adsorption) {
+      // q_ads init: initial value on momentum bounce-back nodes, else 0 (node.rs:77-85)
+      std::vector<double> qa(c->ld, 0.0);
+      for (long long i = 0; i < N; i++) qa[i] = c->h_mask[i] ? sd.initial_q_ads : 0.0;
+      if ((rc = alloc((void**)&S.q_ads, c->ld * 8))) return cleanup(rc);
+      if (hipMemcpy(S.q_ads, qa.data(), c->ld * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(LBF_EDEVICE, "upload of q_ads failed"));
+    }
+    if (sd.source_kind == LBF_SOURCE_FIELD) {
+      std::vector<double> sf(c->ld, 0.0);
+      for (long long i = 0; i < N; i++) sf[i] = sd.source_field[i];
+      if ((rc = alloc((void**)&S.src, c->ld * 8))) return cleanup(rc);
+      if (hipMemcpy(S.src, sf.data(), c->ld * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return cleanup(fail(LBF_EDEVICE, "upload of scalar source failed"));
+    }
+  }
+
+  // ---- base kernel arguments
+  KArgs& a = c->base;
+  a.nx = c->nx, a.ny = c->ny, a.nz = c->nz, a.d = c->d;
+  a.N = N, a.ld = c->ld;
+  a.nfaces = nf;
+  a.face_active = c->face_active;
+  a.cls = c->cls;
+  a.rho[0] = c->rho[0], a.rho[1] = c->rho[1];
+  a.u[0] = c->u[0], a.u[1] = c->u[1];
+  a.partials = c->partials;
+  a.ctrl = c->ctrl;
+  for (int f = 0; f < 6; f++) {
+    a.face[f].kind = f < nf ? d->faces[f].kind : LBF_BC_PERIODIC;
+    a.face[f].known = d->faces[f].known;
+    a.face[f].density = d->faces[f].density;
+    for (int x = 0; x < 3; x++) a.face[f].vel[x] = d->faces[f].velocity[x];
+  }
+  c->desc.node_type = nullptr;
+  c->desc.force_field = nullptr;
+  c->desc.initial_density = nullptr;
+  c->desc.initial_velocity = nullptr;
+  c->desc.scalars = nullptr;
+  *out = c;
+  return 0;
+}
+
+void lbf_destroy(lbf_case* c) {
+  if (!c) return;
+  (void)hipStreamSynchronize(c->stream);
+  free_case(c);
+}
+
+}  // extern "C"
+
+namespace {
+
+KArgs momentum_args(lbf_case* c) {
+  KArgs a = c->base;
+  a.field = 0;
+  a.pop[0] = c->pop[0], a.pop[1] = c->pop[1];
+  fill_collision(a, c->desc.collision, c->q);
+  a.force_kind = c->desc.force_kind;
+  for (int x = 0; x < 3; x++) a.F[x] = c->desc.force[x];
+  a.src_field = c->force;
+  return a;
+}
+
+KArgs scalar_args(lbf_case* c, int s) {
+  KArgs a = c->base;
+  ScalarDev& S = c->sc[s];
+  a.field = 1 + s;
+  a.pop[0] = S.pop[0], a.pop[1] = S.pop[1];
+  a.c[0] = S.c[0], a.c[1] = S.c[1];
+  fill_collision(a, S.desc.collision, c->q);
+  a.force_kind = S.desc.source_kind;  // reused as the scalar source kind
+  a.src_uniform = S.desc.source_uniform;
+  a.src_field = S.src;
+  a.has_ads = S.desc.has_adsorption;
+  a.k_a = S.desc.adsorption_rate, a.k_d = S.desc.desorption_rate, a.q_max = S.desc.q_max;
+  a.q_ads = S.q_ads;
+  a.inner_bc = S.desc.inner_bc;
+  uint32_t act = 0;
+  for (int f = 0; f < 2 * c->d; f++) {
+    a.sface[f].kind = S.desc.faces[f].kind;
+    a.sface[f].unknown_only = S.desc.faces[f].unknown_populations_only;
+    a.sface[f].value = S.desc.faces[f].scalar_value;
+    if (S.desc.faces[f].kind != LBF_SBC_PERIODIC) act |= 1u << f;
+  }
+  a.face_active = act;
+  return a;
+}
+
+uint32_t residual_mask(lbf_case* c) {
+  uint32_t m = 0;
+  if (!c->frozen) {
+    m |= 1u;
+    for (int x = 0; x < c->d; x++) m |= 1u << (1 + x);
+    m |= 1u << 8;
+  }
+  for (int s = 0; s < c->ns; s++) m |= 1u << (4 + s);
+  return m;
+}
+
+// Enqueues one time step: momentum launch, then one launch per scalar; the
+// last launch advances the step.
+int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
+  const bool resid = flags & LBF_STEP_RESIDUALS;
+  const bool stop = flags & LBF_STEP_STOP_ON_CONVERGENCE;
+  uint32_t kf = 0;
+  if (resid) kf |= kFlagResiduals;
+  if (stop) kf |= kFlagStopCheck;
+  const uint32_t mask = residual_mask(c);
+  if (!c->frozen) {
+    KArgs a = momentum_args(c);
+    a.mode = kModePull;
+    a.flags = kf;
+    if (resid || last_of_batch || c->needs_moments) a.flags |= kFlagStoreMoments;
+    if (c->ns == 0) a.flags |= kFlagAdvance;
+    else a.flags &= ~kFlagStopCheck;
+    a.res_mask = mask;
+    HIPCHK(c->L.mom_step(a, c->grid, c->stream));
+  }
+  for (int s = 0; s < c->ns; s++) {
+    KArgs a = scalar_args(c, s);
+    a.mode = kModePull;
+    a.flags = kf;
+    if (s == c->ns - 1) a.flags |= kFlagAdvance;
+    else a.flags &= ~kFlagStopCheck;
+    a.res_mask = mask;
+    HIPCHK(c->L.sca_step(a, c->grid, c->stream));
+  }
+  if (c->frozen && c->ns == 0) {
+    KArgs a = momentum_args(c);
+    a.flags = kf | kFlagAdvance;
+    a.res_mask = 0;
+    HIPCHK(launch_advance_only(a, c->stream));
+  }
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lbf_initialize_nodes(lbf_case* c) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  KArgs a = momentum_args(c);
+  HIPCHK(c->L.mom_init(a, c->grid, c->stream));
+  long long fresh = 1;
+  for (int s = 0; s < c->ns; s++) {
+    KArgs b = scalar_args(c, s);
+    HIPCHK(c->L.sca_init(b, c->grid, c->stream));
+    fresh |= 1ll << (1 + s);
+  }
+  HIPCHK(launch_ctrl_set(c->ctrl, kCtrlFresh, fresh, c->stream));
+  return 0;
+}
+
+int lbf_main_steps(lbf_case* c, uint32_t nsteps, uint32_t flags) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  HIPCHK(launch_ctrl_set(c->ctrl, kCtrlResetStop, 0, c->stream));
+  if (flags & LBF_STEP_STOP_ON_CONVERGENCE)
+    HIPCHK(launch_ctrl_set(c->ctrl, kCtrlMaxIter, -1, c->stream));
+  for (uint32_t k = 0; k < nsteps; k++) {
+    int rc = enqueue_step(c, flags, k + 1 == nsteps);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+int lbf_synchronize(lbf_case* c) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  return sync_and_check(c);
+}
+
+int lbf_run(lbf_case* c, uint64_t max_iterations, uint64_t* steps_done) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  HIPCHK(launch_ctrl_set(c->ctrl, kCtrlResetStop, 0, c->stream));
+  HIPCHK(launch_ctrl_set(c->ctrl, kCtrlMaxIter, (long long)max_iterations, c->stream));
+  const uint32_t flags = LBF_STEP_RESIDUALS | LBF_STEP_STOP_ON_CONVERGENCE;
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  uint64_t chunk = 64;
+  while (true) {
+    // the stop rule fires at the latest at step index max_iterations + 1
+    const uint64_t limit = max_iterations + 2;
+    const uint64_t left = h.step >= limit ? 1 : limit - h.step;
+    const uint64_t n = std::min<uint64_t>(chunk, left);
+    for (uint64_t k = 0; k < n; k++)
+      if ((rc = enqueue_step(c, flags, true))) return rc;
+    if ((rc = read_ctrl(c, &h))) return rc;
+    if (h.stopped) break;
+    chunk = std::min<uint64_t>(chunk * 2, 1024);
+  }
+  if (steps_done) *steps_done = h.step;
+  return 0;
+}
+
+int lbf_residuals(lbf_case* c, double* r_rho, double* r_u, double* r_c) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  if (r_rho) *r_rho = h.res[0];
+  if (r_u)
+    for (int x = 0; x < c->d; x++) r_u[x] = h.res[1 + x];
+  if (r_c)
+    for (int s = 0; s < c->ns; s++) r_c[s] = h.res[4 + s];
+  return 0;
+}
+
+int lbf_time_step(lbf_case* c, uint64_t* t) {
+  if (!c || !t) return fail(LBF_EINVAL, "NULL argument");
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  *t = h.step;
+  return 0;
+}
+
+int lbf_reduce(lbf_case* c, int32_t which, double* out) {
+  if (!c || !out) return fail(LBF_EINVAL, "NULL argument");
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  const int cur = (int)((h.step + 1) & 1);
+  const int kind = which & 0xff, s = which >> 8;
+  double r = 0.0;
+  switch (kind) {
+    case LBF_REDUCE_MEAN_DENSITY:
+    case LBF_REDUCE_TOTAL_MASS:
+      HIPCHK(launch_reduce(c->rho[cur], c->tb == 8, c->cls, 0, c->N, c->partials, c->grid,
+                           c->red_out, 0, c->stream));
+      break;
+    case LBF_REDUCE_SCALAR_TOTAL:
+      if (s >= c->ns) return fail(LBF_EINVAL, "no scalar %d", s);
+      HIPCHK(launch_reduce(c->sc[s].c[cur], c->tb == 8, c->cls, 0, c->N, c->partials, c->grid,
+                           c->red_out, 0, c->stream));
+      break;
+    case LBF_REDUCE_ADSORBED:
+      if (s >= c->ns) return fail(LBF_EINVAL, "no scalar %d", s);
+      if (!c->sc[s].q_ads) {
+        *out = 0.0;
+        return 0;
+      }
+      HIPCHK(launch_reduce(c->sc[s].q_ads, 1, c->cls, 1, c->N, c->partials, c->grid, c->red_out,
+                           0, c->stream));
+      break;
+    case LBF_REDUCE_NONFINITE:
+      HIPCHK(launch_nonfinite(c->rho[cur], c->u[cur], c->tb == 8, c->d, c->ld, c->cls, c->N,
+                              c->partials, c->grid, c->red_out, c->stream));
+      break;
+    default:
+      return fail(LBF_EINVAL, "unknown reduction %d", which);
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(&r, c->red_out, 8, hipMemcpyDeviceToHost));
+  if (kind == LBF_REDUCE_MEAN_DENSITY) {
+    long long nf = 0;
+    for (long long i = 0; i < c->N; i++) nf += c->h_type[i] == 0;
+    r = r / (double)nf;
+  }
+  *out = r;
+  return 0;
+}
+
+int lbf_get_fields(lbf_case* c, double* rho, double* u) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  const int cur = (int)((h.step + 1) & 1);
+  if (rho) {
+    std::vector<double> r(c->ld);
+    if ((rc = download_compute(c, c->rho[cur], r.data(), c->ld))) return rc;
+    memcpy(rho, r.data(), c->N * 8);
+  }
+  if (u) {
+    std::vector<double> uu((size_t)c->d * c->ld);
+    if ((rc = download_compute(c, c->u[cur], uu.data(), uu.size()))) return rc;
+    for (long long i = 0; i < c->N; i++)
+      for (int a = 0; a < c->d; a++) u[i * c->d + a] = uu[a * c->ld + i];
+  }
+  return 0;
+}
+
+int lbf_get_scalar(lbf_case* c, uint32_t s, double* value, double* q_ads) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  if ((int)s >= c->ns) return fail(LBF_EINVAL, "no scalar %u", s);
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  const int cur = (int)((h.step + 1) & 1);
+  if (value) {
+    std::vector<double> v(c->ld);
+    if ((rc = download_compute(c, c->sc[s].c[cur], v.data(), c->ld))) return rc;
+    memcpy(value, v.data(), c->N * 8);
+  }
+  if (q_ads) {
+    if (c->sc[s].q_ads) {
+      HIPCHK(hipMemcpy(q_ads, c->sc[s].q_ads, c->N * 8, hipMemcpyDeviceToHost));
+    } else {
+      for (long long i = 0; i < c->N; i++) q_ads[i] = NAN;
+    }
+  }
+  return 0;
+}
+
+int lbf_get_pops(lbf_case* c, int32_t field, double* out) {
+  if (!c || !out) return fail(LBF_EINVAL, "NULL argument");
+  if (field < 0 || field > c->ns || (field == 0 && c->frozen && false))
+    return fail(LBF_EINVAL, "no field %d", field);
+  const size_t bytes = (size_t)c->q * c->ld * 8;
+  if (!c->readback) HIPCHK(hipMalloc((void**)&c->readback, bytes));
+  HIPCHK(hipMemsetAsync(c->readback, 0, bytes, c->stream));
+  KArgs a = field == 0 ? momentum_args(c) : scalar_args(c, field - 1);
+  a.mode = kModeStreamOnly;
+  a.flags = 0;
+  a.out = c->readback;
+  if (field == 0)
+    HIPCHK(c->L.mom_step(a, c->grid, c->stream));
+  else
+    HIPCHK(c->L.sca_step(a, c->grid, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::vector<double> soa((size_t)c->q * c->ld);
+  HIPCHK(hipMemcpy(soa.data(), c->readback, bytes, hipMemcpyDeviceToHost));
+  for (long long i = 0; i < c->N; i++)
+    for (int j = 0; j < c->q; j++) out[i * c->q + j] = soa[(size_t)j * c->ld + i];
+  return 0;
+}
+
+int lbf_set_pops(lbf_case* c, int32_t field, const double* in) {
+  if (!c || !in) return fail(LBF_EINVAL, "NULL argument");
+  if (field < 0 || field > c->ns) return fail(LBF_EINVAL, "no field %d", field);
+  Ctrl h;
+  int rc = read_ctrl(c, &h);
+  if (rc) return rc;
+  const int par = (int)(h.step & 1);
+  std::vector<double> soa((size_t)c->q * c->ld, 0.0);
+  for (long long i = 0; i < c->N; i++)
+    for (int j = 0; j < c->q; j++) soa[(size_t)j * c->ld + i] = in[i * c->q + j];
+  void* dst = field == 0 ? c->pop[par] : c->sc[field - 1].pop[par];
+  const size_t elem = field == 0 ? c->sb : c->tb;
+  std::vector<char> buf(soa.size() * elem);
+  if (elem == 8) {
+    memcpy(buf.data(), soa.data(), buf.size());
+  } else if (field != 0 || c->prec == LBF_F32) {
+    float* o = reinterpret_cast<float*>(buf.data());
+    for (size_t i = 0; i < soa.size(); i++) o[i] = (float)soa[i];
+  } else {
+    return fail(LBF_EUNSUPPORTED, "set_pops for shifted storage");
+  }
+  HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  HIPCHK(launch_ctrl_set(c->ctrl, kCtrlFresh, 1ll << field, c->stream));
+  return 0;
+}
+
+int lbf_get_link_tables(lbf_case* c, uint8_t* node_type, uint32_t* bb_link_mask,
+                        uint8_t* face_bits) {
+  if (!c) return fail(LBF_EINVAL, "case is NULL");
+  if (node_type) memcpy(node_type, c->h_type.data(), c->N);
+  if (bb_link_mask) memcpy(bb_link_mask, c->h_mask.data(), c->N * 4);
+  if (face_bits) memcpy(face_bits, c->h_faces.data(), c->N);
+  return 0;
+}
+
+}  // extern "C"
+
+// Benchmark hook (not part of the reference-equivalent ABI): the HIP stream
+// a case enqueues its launches on, so callers can bracket work with events.
+extern "C" void* lbfx_case_stream(lbf_case* c) { return c ? (void*)c->stream : nullptr; }

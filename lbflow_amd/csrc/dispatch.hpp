@@ -1,0 +1,41 @@
+// dispatch.hpp — per-lattice launcher tables (one translation unit per lattice
+// so the kernels of the four velocity sets compile in parallel).
+#pragma once
+
+#include "engine.hpp"
+
+namespace lbf {
+
+using LaunchFn = hipError_t (*)(const KArgs&, int grid, hipStream_t);
+
+struct Launchers {
+  LaunchFn mom_step = nullptr;  // fused pull-stream / faces / collide
+  LaunchFn mom_init = nullptr;  // f = f_eq
+  LaunchFn sca_step = nullptr;
+  LaunchFn sca_init = nullptr;
+};
+
+// precision: LBF_F64 .. LBF_BF16_SHIFTED; returns false when not built
+bool launchers_d2q9(int precision, Launchers* out);
+bool launchers_d3q15(int precision, Launchers* out);
+bool launchers_d3q19(int precision, Launchers* out);
+bool launchers_d3q27(int precision, Launchers* out);
+
+// lattice-independent kernels (misc.hip)
+hipError_t launch_ctrl_set(Ctrl* ctrl, int what, long long value, hipStream_t s);
+hipError_t launch_advance_only(const KArgs& a, hipStream_t s);
+hipError_t launch_reduce(const void* values, int is_double, const uint32_t* cls, int need_link,
+                         long long N, double* partials, int grid, double* out, int count_nonfinite,
+                         hipStream_t s);
+
+hipError_t launch_nonfinite(const void* rho, const void* u, int is_double, int d, long long ld,
+                            const uint32_t* cls, long long N, double* partials, int grid,
+                            double* out, hipStream_t s);
+
+enum CtrlOp : int {
+  kCtrlFresh = 0,     // value: field mask to mark fresh
+  kCtrlResetStop = 1,
+  kCtrlMaxIter = 2,   // value: max_iter
+};
+
+}  // namespace lbf

@@ -1,0 +1,65 @@
+// lattice_launch.hpp — instantiates the step kernels of one velocity set.
+#pragma once
+
+#include "dispatch.hpp"
+#include "kernels.hpp"
+
+namespace lbf {
+
+template <class K>
+static hipError_t launch(K kernel, const KArgs& a, int grid, hipStream_t s) {
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+template <class L, class St>
+static hipError_t mom_step_plain(const KArgs& a, int grid, hipStream_t s) {
+  if (a.op == 0) return launch(momentum_step_plain<L, St, 0>, a, grid, s);
+  if (a.op == 1) return launch(momentum_step_plain<L, St, 1>, a, grid, s);
+  if constexpr (L::has_mrt) return launch(momentum_step_plain<L, St, 2>, a, grid, s);
+  return hipErrorInvalidValue;
+}
+template <class L, class St>
+static hipError_t mom_init_t(const KArgs& a, int grid, hipStream_t s) {
+  return launch(momentum_init<L, St>, a, grid, s);
+}
+template <class L, class T>
+static hipError_t sca_step_t(const KArgs& a, int grid, hipStream_t s) {
+  if (a.op == 0) return launch(scalar_step<L, T, 0>, a, grid, s);
+  if (a.op == 1) return launch(scalar_step<L, T, 1>, a, grid, s);
+  if constexpr (L::has_mrt) return launch(scalar_step<L, T, 2>, a, grid, s);
+  return hipErrorInvalidValue;
+}
+template <class L, class T>
+static hipError_t sca_init_t(const KArgs& a, int grid, hipStream_t s) {
+  return launch(scalar_init<L, T>, a, grid, s);
+}
+
+template <class L>
+static bool make_launchers(int precision, Launchers* out) {
+  switch (precision) {
+    case 0:  // LBF_F64
+      out->mom_step = mom_step_plain<L, StF64>;
+      out->mom_init = mom_init_t<L, StF64>;
+      out->sca_step = sca_step_t<L, double>;
+      out->sca_init = sca_init_t<L, double>;
+      return true;
+    case 1:  // LBF_F32
+      out->mom_step = mom_step_plain<L, StF32>;
+      out->mom_init = mom_init_t<L, StF32>;
+      out->sca_step = sca_step_t<L, float>;
+      out->sca_init = sca_init_t<L, float>;
+      return true;
+    default:
+      return false;
+  }
+}
+
+}  // namespace lbf
+
+#define LBF_LATTICE_TU(Lat, name)                              \
+  namespace lbf {                                              \
+  bool launchers_##name(int precision, Launchers* out) {       \
+    return make_launchers<Lat>(precision, out);                \
+  }                                                            \
+  }

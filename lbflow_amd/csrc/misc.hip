@@ -1,0 +1,116 @@
+// misc.hip — lattice-independent kernels: control-block updates, the
+// advance-only step (frozen momentum without scalars) and FP64 reductions.
+#include <cmath>
+
+#include "dispatch.hpp"
+#include "kernels.hpp"
+
+namespace lbf {
+
+__global__ void ctrl_set_kernel(Ctrl* ctrl, int what, long long value) {
+  if (threadIdx.x != 0) return;
+  switch (what) {
+    case kCtrlFresh:
+      for (int f = 0; f < 5; f++)
+        if (value >> f & 1) ctrl->fresh[f] = 1;
+      break;
+    case kCtrlResetStop:
+      ctrl->stopped = 0;
+      ctrl->done = 0;
+      break;
+    case kCtrlMaxIter:
+      ctrl->max_iter = (unsigned long long)value;
+      break;
+  }
+}
+
+hipError_t launch_ctrl_set(Ctrl* ctrl, int what, long long value, hipStream_t s) {
+  hipLaunchKernelGGL(ctrl_set_kernel, dim3(1), dim3(64), 0, s, ctrl, what, value);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(kBlock) advance_only_kernel(KArgs a) {
+  if (a.ctrl->stopped) return;
+  const unsigned long long step = a.ctrl->step;
+  advance_step(a, step);
+}
+
+hipError_t launch_advance_only(const KArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(advance_only_kernel, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+// Sum of values[n] over fluid nodes (need_link: only nodes with a solid
+// neighbour). count_nonfinite: counts non-finite values instead. Two passes
+// with fixed association order, so the result does not depend on timing.
+template <class T>
+__global__ void __launch_bounds__(kBlock) reduce_pass1(const T* values, const uint32_t* cls,
+                                                       int need_link, long long N, int count_nf,
+                                                       double* partials) {
+  __shared__ double lds[kBlock / 64];
+  double acc = 0.0;
+  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < N;
+       n += (long long)gridDim.x * kBlock) {
+    const uint32_t c = cls[n];
+    if (c & kSolidBit) continue;
+    if (need_link && !(c & kLinkMask)) continue;
+    const double v = (double)values[n];
+    acc += count_nf ? (std::isfinite(v) ? 0.0 : 1.0) : v;
+  }
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(kBlock) reduce_pass2(const double* partials, int n, double* out) {
+  __shared__ double lds[kBlock / 64];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n; b += kBlock) acc += partials[b];
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) *out = s;
+}
+
+// number of fluid nodes with a non-finite density or velocity component
+template <class T>
+__global__ void __launch_bounds__(kBlock) nonfinite_pass1(const T* rho, const T* u, int d,
+                                                          long long ld, const uint32_t* cls,
+                                                          long long N, double* partials) {
+  __shared__ double lds[kBlock / 64];
+  double acc = 0.0;
+  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < N;
+       n += (long long)gridDim.x * kBlock) {
+    if (cls[n] & kSolidBit) continue;
+    bool bad = !std::isfinite((double)rho[n]);
+    for (int x = 0; x < d; x++) bad = bad || !std::isfinite((double)u[x * ld + n]);
+    acc += bad ? 1.0 : 0.0;
+  }
+  const double s = block_sum(acc, lds);
+  if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+hipError_t launch_nonfinite(const void* rho, const void* u, int is_double, int d, long long ld,
+                            const uint32_t* cls, long long N, double* partials, int grid,
+                            double* out, hipStream_t s) {
+  if (is_double)
+    hipLaunchKernelGGL(nonfinite_pass1<double>, dim3(grid), dim3(kBlock), 0, s,
+                       (const double*)rho, (const double*)u, d, ld, cls, N, partials);
+  else
+    hipLaunchKernelGGL(nonfinite_pass1<float>, dim3(grid), dim3(kBlock), 0, s, (const float*)rho,
+                       (const float*)u, d, ld, cls, N, partials);
+  hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_reduce(const void* values, int is_double, const uint32_t* cls, int need_link,
+                         long long N, double* partials, int grid, double* out, int count_nonfinite,
+                         hipStream_t s) {
+  if (is_double)
+    hipLaunchKernelGGL(reduce_pass1<double>, dim3(grid), dim3(kBlock), 0, s,
+                       (const double*)values, cls, need_link, N, count_nonfinite, partials);
+  else
+    hipLaunchKernelGGL(reduce_pass1<float>, dim3(grid), dim3(kBlock), 0, s, (const float*)values,
+                       cls, need_link, N, count_nonfinite, partials);
+  hipLaunchKernelGGL(reduce_pass2, dim3(1), dim3(kBlock), 0, s, partials, grid, out);
+  return hipGetLastError();
+}
+
+}  // namespace lbf

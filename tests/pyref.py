@@ -1,0 +1,344 @@
+"""A second, independent restatement of the reference step in pure Python.
+
+Written node-object style like the reference (one object per node with f,
+f_eq, f_star lists and a neighbour dict, src/momentum/node.rs:40-58,
+src/momentum/lattice.rs:127-191) and run phase by phase like
+src/momentum/lattice.rs:431-441 and src/passive_scalar/lattice.rs:342-350.
+Python floats are IEEE doubles without fused multiply-add, so with the same
+association order this must agree with the C oracle bit for bit; tests use it
+on tiny lattices to catch transcription errors in either restatement.
+Only the tests import this module.
+"""
+from __future__ import annotations
+
+import math
+
+from lbflow_amd import abi
+
+CS2I, CS4I = 3.0, 9.0
+
+
+def vs_tables(vs):
+    from oracle import oracle
+    t = oracle.tables(vs)
+    c = [list(map(int, row[: t["d"]])) for row in t["c"]]
+    return t["d"], t["q"], c, list(map(float, t["w"])), list(map(int, t["qbar"])), \
+        [list(map(int, r)) for r in t["qfaces"]], list(map(int, t["normal"]))
+
+
+def fsum(xs):
+    it = iter(xs)
+    s = next(it)
+    for x in it:
+        s = s + x
+    return s
+
+
+class Node:
+    pass
+
+
+class PyLattice:
+    def __init__(self, spec):
+        self.spec = spec
+        vs = spec.velocity_set
+        self.d, self.q, self.c, self.w, self.qbar, self.qfaces, self.normal = vs_tables(vs)
+        if spec.collision.op == abi.MRT or any(s.collision.op == abi.MRT for s in spec.scalars):
+            from oracle import oracle
+            M, Mi = oracle.mrt_matrices(vs)
+            self.M, self.Mi = M.tolist(), Mi.tolist()
+        n = list(spec.shape) + [1] * (3 - spec.d)
+        self.n = n
+        N = n[0] * n[1] * n[2]
+        nt = spec.node_type if spec.node_type is not None else [0] * N
+        self.nodes = []
+        for i in range(N):
+            nd = Node()
+            nd.i = i
+            nd.idx = (i % n[0], (i // n[0]) % n[1], i // (n[0] * n[1]))
+            nd.solid = bool(nt[i])
+            rho0 = spec.initial_density if not hasattr(spec.initial_density, "__len__") else \
+                spec.initial_density[i]
+            nd.rho = float(rho0)
+            if spec.initial_velocity is None:
+                nd.u = [0.0] * self.d
+            elif len(spec.initial_velocity) == self.d and not hasattr(spec.initial_velocity[0], "__len__"):
+                nd.u = [float(v) for v in spec.initial_velocity]
+            else:
+                nd.u = [float(v) for v in spec.initial_velocity[i]]
+            nd.f = [0.0] * self.q
+            nd.feq = [0.0] * self.q
+            nd.fs = [0.0] * self.q
+            nd.sh_rho, nd.sh_u = nd.rho, list(nd.u)
+            self.nodes.append(nd)
+        for nd in self.nodes:
+            nd.nbr = {}
+            for k in range(self.q):
+                j = [(nd.idx[a] + (self.c[k][a] if a < self.d else 0)) % n[a] for a in range(3)]
+                nd.nbr[k] = self.nodes[j[0] + n[0] * (j[1] + n[1] * j[2])]
+        self.fluid = [nd for nd in self.nodes if not nd.solid]
+        self.faces = []
+        for f in range(2 * self.d):
+            axis, lim = f // 2, (0 if f % 2 == 0 else n[f // 2] - 1)
+            self.faces.append([nd for nd in self.fluid if nd.idx[axis] == lim])
+        self.bb = []
+        for nd in self.fluid:
+            nd.bbn = {k: v for k, v in nd.nbr.items() if v.solid}
+            if nd.bbn:
+                self.bb.append(nd)
+        # scalars
+        self.sc = []
+        for s, sp in enumerate(spec.scalars):
+            S = {"spec": sp, "g": {}, "geq": {}, "gs": {}, "c": {}, "csh": {}, "q": {}}
+            for nd in self.nodes:
+                iv = sp.initial_value if not hasattr(sp.initial_value, "__len__") else sp.initial_value[nd.i]
+                S["c"][nd.i] = float(iv)
+                S["csh"][nd.i] = float(iv)
+                S["g"][nd.i] = [0.0] * self.q
+                S["geq"][nd.i] = [0.0] * self.q
+                S["gs"][nd.i] = [0.0] * self.q
+                if sp.adsorption:
+                    S["q"][nd.i] = float(sp.adsorption.get("initial_q_ads", 0.0)) if nd in self.bb else 0.0
+            self.sc.append(S)
+        self.t = 0
+
+    # -- kernel.rs
+    def eq(self, value, u):
+        udu = fsum(x * x for x in u)
+        out = []
+        for i in range(self.q):
+            udc = fsum(u[a] * float(self.c[i][a]) for a in range(self.d))
+            out.append(self.w[i] * value * (1.0 + udc * CS2I + 0.5 * udc * udc * CS4I - 0.5 * udu * CS2I))
+        return out
+
+    def bgk(self, f, feq, tau):
+        om = 1.0 / tau
+        omp = 1.0 - om
+        return [omp * f[i] + om * feq[i] for i in range(self.q)]
+
+    def trt(self, f, feq, wp, wm):
+        out = []
+        for i in range(self.q):
+            ib = self.qbar[i]
+            fp = 0.5 * (f[i] + f[ib])
+            fep = 0.5 * (feq[i] + feq[ib])
+            fm = 0.5 * (f[i] - f[ib])
+            fem = 0.5 * (feq[i] - feq[ib])
+            out.append(f[i] - wp * 1.0 * (fp - fep) - wm * 1.0 * (fm - fem))
+        return out
+
+    def meq(self, rho, u):
+        from oracle import oracle
+        return oracle.k_meq(self.spec.velocity_set, rho, u).tolist()
+
+    def mrt(self, value, u, f, s):
+        q = self.q
+        m = [fsum(self.M[k][i] * f[i] for i in range(q)) for k in range(q)]
+        me = self.meq(value, u)
+        ms = [m[k] - s[k] * 1.0 * (m[k] - me[k]) for k in range(q)]
+        return [fsum(self.Mi[i][k] * ms[k] for k in range(q)) for i in range(q)]
+
+    def guo(self, u, F, tau):
+        cb = 1.0 - 0.5 * 1.0 / tau
+        out = []
+        for i in range(self.q):
+            udc = fsum(u[a] * float(self.c[i][a]) for a in range(self.d))
+            acc = fsum((CS2I * float(self.c[i][a]) - CS2I * u[a] + CS4I * udc * float(self.c[i][a])) * F[a]
+                       for a in range(self.d))
+            out.append(cb * self.w[i] * acc)
+        return out
+
+    def force(self, nd):
+        sp = self.spec
+        if sp.force is None:
+            return None
+        if len(sp.force) == self.d and not hasattr(sp.force[0], "__len__"):
+            return [float(v) for v in sp.force]
+        return [float(v) for v in sp.force[nd.i]]
+
+    # -- momentum phases (lattice.rs:431-441)
+    def momentum_step(self):
+        sp = self.spec
+        for nd in self.fluid:
+            nd.rho = fsum(nd.f)
+            nd.u = [fsum(nd.f[i] * float(self.c[i][a]) for i in range(self.q)) / nd.rho
+                    for a in range(self.d)]
+            F = self.force(nd)
+            if F is not None:
+                nd.u = [nd.u[a] + 0.5 * 1.0 * F[a] / nd.rho for a in range(self.d)]
+        for nd in self.fluid:
+            nd.feq = self.eq(nd.rho, nd.u)
+        col = sp.collision
+        for nd in self.fluid:
+            if col.op == abi.BGK:
+                fs = self.bgk(nd.f, nd.feq, col.tau)
+                F = self.force(nd)
+                if F is not None:
+                    S = self.guo(nd.u, F, col.tau)
+                    fs = [fs[i] + S[i] for i in range(self.q)]
+            elif col.op == abi.TRT:
+                fs = self.trt(nd.f, nd.feq, col.omega_plus, col.omega_minus)
+            else:
+                fs = self.mrt(nd.rho, nd.u, nd.f, list(col.relaxation))
+            nd.fs = fs
+        for nd in self.fluid:
+            f = [0.0] * self.q
+            for i, nb in nd.nbr.items():
+                f[self.qbar[i]] = nb.fs[self.qbar[i]]
+            nd.f = f
+        for nd in self.bb:
+            for i in nd.bbn:
+                nd.f[self.qbar[i]] = nd.fs[i]
+        for fc in range(2 * self.d):
+            bc = sp.faces[fc]
+            for nd in self.faces[fc]:
+                if bc.kind == abi.BC_NO_SLIP:
+                    for i in self.qfaces[fc]:
+                        nd.f[self.qbar[i]] = nd.fs[i]
+                elif bc.kind == abi.BC_BOUNCE_BACK:
+                    for i in self.qfaces[fc]:
+                        udc = fsum(float(bc.velocity[a]) * float(self.c[i][a]) for a in range(self.d))
+                        nd.f[self.qbar[i]] = nd.fs[i] - 2.0 * self.w[i] * bc.density * CS2I * udc
+                elif bc.kind == abi.BC_ZOU_HE:
+                    from oracle import oracle
+                    known = bc.to_c().known
+                    vel = [v if v is not None else 0.0 for v in list(bc.velocity)[:3]]
+                    rc, f = oracle.k_zou_he(fc, nd.f, known, bc.density or 0.0, vel)
+                    assert rc == 0
+                    nd.f = f.tolist()
+                elif bc.kind == abi.BC_PERIODIC:
+                    pass
+                else:
+                    raise NotImplementedError(bc.kind)
+
+    # -- scalar (passive_scalar/node.rs, adsorption.rs)
+    def source(self, S, nd):
+        sp = S["spec"]
+        user = None
+        if sp.source is not None:
+            user = float(sp.source) if not hasattr(sp.source, "__len__") else float(sp.source[nd.i])
+        ads = None
+        if sp.adsorption:
+            if nd in self.bb:
+                c = S["c"][nd.i]
+                q = S["q"][nd.i]
+                ka, kd, qm = sp.adsorption["adsorption_rate"], sp.adsorption.get("desorption_rate", 0.0), \
+                    sp.adsorption["q_max"]
+                r = ka * c * (qm - q) - kd * q
+                ads = -r
+                qn = q + r * 1.0
+                qn = 0.0 if qn < 0.0 else (qm if qn > qm else qn)
+                S["q"][nd.i] = qn
+            else:
+                ads = 0.0
+        if user is not None and ads is not None:
+            return user + ads
+        return user if user is not None else ads
+
+    def scalar_step(self, S):
+        sp = S["spec"]
+        for nd in self.fluid:
+            c = fsum(S["g"][nd.i])
+            src = self.source(S, nd)
+            if src is not None:
+                c += 0.5 * 1.0 * src
+            S["c"][nd.i] = c
+        for nd in self.fluid:
+            S["geq"][nd.i] = self.eq(S["c"][nd.i], nd.u)
+        col = sp.collision
+        for nd in self.fluid:
+            g, geq = S["g"][nd.i], S["geq"][nd.i]
+            if col.op == abi.BGK:
+                gs = self.bgk(g, geq, col.tau)
+                src = self.source(S, nd)
+                if src is not None:
+                    cb = 1.0 - 0.5 / col.tau
+                    gs = [gs[i] + cb * self.w[i] * src for i in range(self.q)]
+            elif col.op == abi.TRT:
+                gs = self.trt(g, geq, col.omega_plus, col.omega_minus)
+            else:
+                gs = self.mrt(S["c"][nd.i], nd.u, g, list(col.relaxation))
+            S["gs"][nd.i] = gs
+        for nd in self.fluid:
+            g = [0.0] * self.q
+            for i, nb in nd.nbr.items():
+                g[self.qbar[i]] = S["gs"][nb.i][self.qbar[i]]
+            S["g"][nd.i] = g
+        for nd in self.bb:
+            for i, sn in nd.bbn.items():
+                if sp.inner_bc == abi.INNER_BOUNCE_BACK:
+                    S["g"][nd.i][self.qbar[i]] = S["gs"][nd.i][i]
+                else:
+                    S["g"][nd.i][self.qbar[i]] = -S["gs"][nd.i][i] + 2.0 * self.w[i] * S["c"][sn.i]
+        for fc in range(2 * self.d):
+            bc = sp.faces[fc]
+            mb = self.spec.faces[fc]
+            if mb.kind == abi.BC_NO_SLIP:
+                vel = [0.0] * self.d
+            elif mb.kind == abi.BC_BOUNCE_BACK:
+                vel = [float(v) for v in mb.velocity[: self.d]]
+            elif mb.kind == abi.BC_ZOU_HE and all(v is not None for v in list(mb.velocity)[: self.d]):
+                vel = [float(v) for v in mb.velocity[: self.d]]
+            else:
+                vel = None
+            for nd in self.faces[fc]:
+                g, gs = S["g"][nd.i], S["gs"][nd.i]
+                if bc.kind in (abi.SBC_ANTI_BOUNCE_BACK, abi.SBC_ANTI_BB_NO_FLUX):
+                    value = bc.scalar_value if bc.kind == abi.SBC_ANTI_BOUNCE_BACK else S["c"][nd.i]
+                    if vel is None:
+                        nb = nd.nbr[self.normal[fc]]
+                        v = [1.5 * nd.u[a] - 0.5 * nb.u[a] for a in range(self.d)]
+                    else:
+                        v = vel
+                    udu = fsum(x * x for x in v)
+                    for i in self.qfaces[fc]:
+                        udc = fsum(v[a] * float(self.c[i][a]) for a in range(self.d))
+                        ge = self.w[i] * value * (1.0 + udc * CS2I + 0.5 * udc * udc * CS4I - 0.5 * udu * CS2I)
+                        g[self.qbar[i]] = -gs[i] + 2.0 * ge
+                elif bc.kind == abi.SBC_BB_NO_FLUX:
+                    for i in self.qfaces[fc]:
+                        g[self.qbar[i]] = gs[i]
+                elif bc.kind == abi.SBC_ZEROTH_ORDER_NO_FLUX:
+                    ng = S["g"][nd.nbr[self.normal[fc]].i]
+                    if bc.unknown_populations_only:
+                        for i in self.qfaces[fc]:
+                            g[self.qbar[i]] = ng[self.qbar[i]]
+                    else:
+                        S["g"][nd.i] = list(ng)
+                elif bc.kind == abi.SBC_SECOND_ORDER_NO_FLUX:
+                    nb = nd.nbr[self.normal[fc]]
+                    nn = nb.nbr[self.normal[fc]]
+                    ng, nng = S["g"][nb.i], S["g"][nn.i]
+                    idx = [self.qbar[i] for i in self.qfaces[fc]] if bc.unknown_populations_only \
+                        else range(self.q)
+                    for i in idx:
+                        g[i] = 2.0 * ng[i] - nng[i]
+
+    def initialize_nodes(self):
+        for nd in self.fluid:
+            nd.feq = self.eq(nd.rho, nd.u)
+            nd.f = list(nd.feq)
+        for S in self.sc:
+            for nd in self.fluid:
+                S["geq"][nd.i] = self.eq(S["c"][nd.i], nd.u)
+                S["g"][nd.i] = list(S["geq"][nd.i])
+
+    def main_steps(self, n=1):
+        for _ in range(n):
+            if not self.spec.freeze_momentum:
+                self.momentum_step()
+            for S in self.sc:
+                self.scalar_step(S)
+            for nd in self.fluid:
+                nd.sh_rho, nd.sh_u = nd.rho, list(nd.u)
+            self.t += 1
+
+    def residuals(self):
+        r = math.sqrt(sum((nd.rho - nd.sh_rho) ** 2 for nd in self.fluid))
+        return r
+
+    def pops(self):
+        return [list(nd.f) for nd in self.nodes]
+
+    def fields(self):
+        return [nd.rho for nd in self.nodes], [list(nd.u) for nd in self.nodes]

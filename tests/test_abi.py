@@ -1,0 +1,72 @@
+"""CPU checks of the C ABI: the ctypes mirror matches include/lbflow_hip.h,
+the built library loads and exports every function the header declares,
+and descriptor validation errors come back as codes (no compute calls)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from lbflow_amd import abi, configs
+from lbflow_amd.engine import library_path, load_library
+from oracle import oracle
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            text = open(os.path.join(ROOT, "include", fn)).read()
+            text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+            names |= set(re.findall(r"\b(lbf_\w+)\s*\(", text))
+    return names
+
+
+def test_struct_layout_matches_header():
+    oracle.build()
+    lib = oracle.load()
+    assert lib.lbo_sizeof(0) == C.sizeof(abi.CaseDesc)
+    assert lib.lbo_sizeof(1) == C.sizeof(abi.ScalarDesc)
+    assert lib.lbo_sizeof(2) == C.sizeof(abi.FaceBC)
+    assert lib.lbo_sizeof(3) == C.sizeof(abi.Collision)
+    assert lib.lbo_sizeof(4) == C.sizeof(abi.ScalarFaceBC)
+    assert lib.lbo_sizeof(5) == abi.CaseDesc.scalars.offset
+    assert lib.lbo_sizeof(6) == abi.CaseDesc.shift_scale.offset
+    assert lib.lbo_sizeof(7) == abi.ScalarDesc.initial_q_ads.offset
+    assert lib.lbo_sizeof(8) == abi.CaseDesc.force_field.offset
+
+
+def test_entry_point_list_equals_header():
+    assert header_functions() == {n for n, _, _ in abi.ENTRY_POINTS}
+
+
+def test_library_exports_every_declared_function():
+    assert os.path.exists(library_path()), "build the engine first (python -m lbflow_amd.build)"
+    lib = load_library()
+    for name in header_functions():
+        assert hasattr(lib, name), name
+
+
+def test_validation_errors_without_gpu():
+    lib = load_library()
+    bad = configs.cavity(n=8)
+    desc, keep = bad.to_c()
+    desc.abi_version = 99
+    out = C.c_uint64()
+    assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == abi.EINVAL
+    assert b"ABI" in lib.lbf_last_error()
+    zh = configs.cavity(n=8)
+    zh.faces[0] = zh.faces[0].zou_he(1.0, (0.1, 0.0))  # density and ux both given: reference panics
+    desc, keep = zh.to_c()
+    assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == abi.EINVAL
+    d3 = configs.porous_channel(n=8)
+    d3.faces[0] = d3.faces[0].zou_he(None, (0.1, 0.0, 0.0))
+    desc, keep = d3.to_c()
+    assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == abi.EUNSUPPORTED
+    ok = configs.porous_channel(n=16, precision=abi.F32)
+    desc, keep = ok.to_c()
+    assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == 0
+    assert out.value >= 2 * 19 * 16 ** 3 * 4

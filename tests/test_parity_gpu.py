@@ -1,0 +1,148 @@
+"""GPU parity: the HIP engine (through the C ABI) against the CPU oracle.
+
+f64 storage: bit-exact node classes and link masks; populations, moments,
+scalar values and adsorbed amounts within 1e-12 relative after N steps
+(in practice the kernels keep the reference's association order and agree
+to the last bit on most cases). Residual norms: 1e-12 relative (their sum
+order is not pinned by the reference, which uses rayon).
+"""
+import numpy as np
+import pytest
+
+from lbflow_amd import abi, configs, geometry
+from lbflow_amd.case import Collision, FaceBC, ScalarFaceBC, ScalarSpec, Spec
+from lbflow_amd.engine import Case
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() / scale
+
+
+def cases():
+    mask2 = geometry.bernoulli_mask((17, 12), 0.2, 5)
+    two = configs.cavity(n=17, tau=0.7).replace(n=(17, 12), node_type=mask2)
+    two.faces = [FaceBC.bounce_back(1.0, (0.0, 0.02)), FaceBC.no_slip(), FaceBC.periodic(),
+                 FaceBC.periodic()]
+    two.scalars = [ScalarSpec(collision=Collision.bgk(0.8),
+                              faces=[ScalarFaceBC(abi.SBC_ANTI_BB_NO_FLUX),
+                                     ScalarFaceBC(abi.SBC_SECOND_ORDER_NO_FLUX),
+                                     ScalarFaceBC(abi.SBC_PERIODIC), ScalarFaceBC(abi.SBC_PERIODIC)],
+                              inner_bc=abi.INNER_ANTI_BOUNCE_BACK,
+                              initial_value=np.linspace(0, 1, 17 * 12), source=1e-3,
+                              adsorption=dict(adsorption_rate=0.1, desorption_rate=0.01, q_max=0.5,
+                                              initial_q_ads=0.05)),
+                   ScalarSpec(collision=Collision.trt(1.1, 0.8),
+                              faces=[ScalarFaceBC(abi.SBC_ANTI_BOUNCE_BACK, 0.3),
+                                     ScalarFaceBC(abi.SBC_ZEROTH_ORDER_NO_FLUX),
+                                     ScalarFaceBC(abi.SBC_PERIODIC), ScalarFaceBC(abi.SBC_PERIODIC)],
+                              inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.2,
+                              adsorption=dict(adsorption_rate=0.2, q_max=1.0))]
+    anti = configs.cavity(n=12, tau=0.9)
+    anti.faces = [FaceBC.periodic(), FaceBC.periodic(), FaceBC.anti_bounce_back(1.01),
+                  FaceBC.no_slip()]
+    return {
+        "cavity_bgk_32": (configs.cavity(n=32, tau=0.6), 40),
+        "cavity_trt": (configs.cavity(n=24, op=Collision.trt(1.2, 0.9)), 30),
+        "cavity_mrt": (configs.cavity(n=20, op=Collision.mrt([1.0, 1.4, 1.2, 1.0, 1.1, 1.0, 1.1, 1.3, 1.3])), 30),
+        "porous_trt_force_32": (configs.porous_channel(n=32, force=1e-4), 20),
+        "porous_bgk_guo": (configs.porous_channel(n=(20, 12, 16), op=Collision.bgk(0.7), force=1e-4, seed=7), 20),
+        "porous_mrt_d3q19": (configs.porous_channel(n=(12, 10, 9), op=Collision.mrt([1.0, 1.19, 1.4, 1.0, 1.2, 1.0, 1.2, 1.0, 1.2, 1.25, 1.4, 1.25, 1.4, 1.25, 1.25, 1.25, 1.98, 1.98, 1.98]), force=1e-4), 15),
+        "d3q15_mrt_periodic": (configs.periodic_box(n=(10, 9, 8), velocity_set=abi.D3Q15, op=Collision.mrt([1.0, 1.2, 1.3, 1.0, 1.1, 1.0, 1.1, 1.0, 1.1, 1.25, 1.25, 1.25, 1.25, 1.25, 1.1])).replace(initial_velocity=np.random.default_rng(0).uniform(-0.02, 0.02, (720, 3))), 15),
+        "d3q27_bgk_porous": (configs.porous_channel(n=(9, 8, 7), velocity_set=abi.D3Q27, op=Collision.bgk(0.8), force=1e-4), 12),
+        "coupled_discs": (configs.coupled_discs(nx=48, ny=24, count=6, rmin=1.5, rmax=3, seed=1, precision=abi.F64, k_a=0.05), 40),
+        "coupled_discs_unknown_only": (configs.coupled_discs(nx=40, ny=20, count=5, rmin=1.5, rmax=3, seed=2, precision=abi.F64, k_a=0.05, zeroth_unknown_only=True), 30),
+        "two_scalars_mixed_bcs": (two, 25),
+        "anti_bounce_back_walls": (anti, 20),
+    }
+
+
+@pytest.mark.parametrize("name", list(cases()))
+def test_f64_parity_with_oracle(name):
+    sp, steps = cases()[name]
+    print(f"[parity] {name} shape={sp.shape} steps={steps}", flush=True)
+    g = Case(sp)
+    o = oracle.OracleCase(sp)
+    t1, m1, f1 = g.link_tables()
+    t2, m2, f2 = o.link_tables()
+    assert np.array_equal(t1, t2) and np.array_equal(m1, m2) and np.array_equal(f1, f2)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    assert rel_err(g.pops(), o.pops()) <= RTOL
+    half = steps // 2
+    g.main_steps(half, residuals=True)
+    o.main_steps(half, residuals=True)
+    for k in range(2):
+        assert g.time_step() == o.time_step()
+        assert rel_err(g.pops(), o.pops()) <= RTOL, f"{name}: momentum populations"
+        rg, ug = g.fields()
+        ro, uo = o.fields()
+        assert rel_err(rg, ro) <= RTOL and rel_err(ug, uo) <= RTOL
+        r1, u1, c1 = g.residuals()
+        r2, u2, c2 = o.residuals()
+        assert abs(r1 - r2) <= RTOL * max(abs(r2), 1e-300) + 1e-300
+        assert np.all(np.abs(u1 - u2) <= RTOL * np.maximum(np.abs(u2), 1e-300) + 1e-300)
+        for s in range(len(sp.scalars)):
+            cg, qg = g.scalar(s)
+            co, qo = o.scalar(s)
+            assert rel_err(cg, co) <= RTOL, f"{name}: scalar {s}"
+            assert rel_err(g.pops(1 + s), o.pops(1 + s)) <= RTOL
+            if sp.scalars[s].adsorption:
+                assert rel_err(qg, qo) <= RTOL
+                assert abs(g.reduce(abi.REDUCE_ADSORBED, s) - o.reduce(abi.REDUCE_ADSORBED, s)) <= \
+                    1e-12 * max(1.0, abs(o.reduce(abi.REDUCE_ADSORBED, s)))
+            assert abs(c1[s] - c2[s]) <= RTOL * max(abs(c2[s]), 1e-300) + 1e-300
+        assert abs(g.reduce(abi.REDUCE_MEAN_DENSITY) - o.reduce(abi.REDUCE_MEAN_DENSITY)) < 1e-13
+        assert g.reduce(abi.REDUCE_NONFINITE) == 0
+        g.main_steps(steps - half)
+        o.main_steps(steps - half)
+
+
+def test_run_stop_rule_matches_oracle():
+    sp = configs.cavity(n=10, tau=0.6)
+    g, o = Case(sp), oracle.OracleCase(sp)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    assert g.run(7) == o.run(7) == 9
+    assert rel_err(g.pops(), o.pops()) <= RTOL
+
+
+def test_convergence_stop_matches_oracle():
+    # a rest state converges at once: the loop stops at t = MIN_ITER + 1
+    sp = configs.periodic_box(n=(6, 5, 4))
+    g, o = Case(sp), oracle.OracleCase(sp)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    assert g.run(1000) == o.run(1000) == 12
+
+
+def test_set_pops_roundtrip_and_continue():
+    sp = configs.porous_channel(n=(10, 8, 6), force=1e-4)
+    g, o = Case(sp), oracle.OracleCase(sp)
+    rng = np.random.default_rng(3)
+    f = oracle.k_equilibrium(abi.D3Q19, 1.0, [0.01, 0, 0])[None, :] * (1 + rng.uniform(-0.01, 0.01, (480, 19)))
+    f[sp.node_type.astype(bool)] = 0.0
+    g.set_pops(f)
+    o.set_pops(f)
+    assert np.array_equal(g.pops(), o.pops())
+    g.main_steps(5)
+    o.main_steps(5)
+    assert rel_err(g.pops(), o.pops()) <= RTOL
+
+
+def test_freeze_momentum():
+    sp = configs.coupled_discs(nx=24, ny=12, count=3, rmin=1.5, rmax=2.5, seed=3, precision=abi.F64)
+    sp = sp.replace(freeze_momentum=True, initial_velocity=(0.02, 0.0))
+    g, o = Case(sp), oracle.OracleCase(sp)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    g.main_steps(10, residuals=True)
+    o.main_steps(10, residuals=True)
+    assert rel_err(g.scalar(0)[0], o.scalar(0)[0]) <= RTOL
+    assert g.residuals()[0] == 0.0
