@@ -44,6 +44,9 @@ class Hip:
         L.hipEventSynchronize.argtypes = [C.c_void_p]
         L.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
         L.hipEventDestroy.argtypes = [C.c_void_p]
+        L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        L.hipFree.argtypes = [C.c_void_p]
+        L.hipDeviceSynchronize.argtypes = []
 
     def event(self):
         e = C.c_void_p()
@@ -58,6 +61,28 @@ class Hip:
         ms = C.c_float()
         assert self.lib.hipEventElapsedTime(C.byref(ms), a, b) == 0
         return ms.value
+
+
+def copy_bandwidth(hip, mib=2048, reps=10):
+    """Device-to-device copy rate (GB/s counted as bytes read + written)."""
+    from lbflow_amd.engine import load_library
+    lib = load_library()
+    lib.lbfx_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]
+    nbytes = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.lib.hipMalloc(C.byref(a), nbytes) == 0
+    assert hip.lib.hipMalloc(C.byref(b), nbytes) == 0
+    for _ in range(2):
+        lib.lbfx_copy(b, a, nbytes, None)
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, None)
+    for _ in range(reps):
+        lib.lbfx_copy(b, a, nbytes, None)
+    hip.record(e1, None)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.lib.hipFree(a)
+    hip.lib.hipFree(b)
+    return 2 * nbytes * reps / (ms / 1e3) / 1e9
 
 
 def time_case(spec, steps, warmup, hip):
@@ -127,13 +152,20 @@ def main():
     hip = Hip()
     n = args.n
     N = n ** 3
-    variants = [("f32", abi.F32, 4), ("f64", abi.F64, 8)]
+    variants = [("f32", abi.F32, abi.STREAM_AA, 4), ("f64", abi.F64, abi.STREAM_TWO_ARRAY, 8),
+                ("f16s", abi.F16_SHIFTED, abi.STREAM_AA, 2),
+                ("bf16s", abi.BF16_SHIFTED, abi.STREAM_AA, 2),
+                ("f32s", abi.F32_SHIFTED, abi.STREAM_AA, 4),
+                ("f32_two_array", abi.F32, abi.STREAM_TWO_ARRAY, 4),
+                ("f64_aa", abi.F64, abi.STREAM_AA, 8)]
     if args.only:
         keep = set(args.only.split(","))
         variants = [v for v in variants if v[0] in keep]
     results = {}
-    for name, prec, sbytes in variants:
-        sp = configs.porous_channel(n=n, precision=prec)
+    copy_gbs = copy_bandwidth(hip)
+    print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    for name, prec, streaming, sbytes in variants:
+        sp = configs.porous_channel(n=n, precision=prec, streaming=streaming)
         print(f"[bench] {name}: D3Q19 TRT porous {n}^3, warmup {args.warmup}, steps {args.steps}",
               file=sys.stderr, flush=True)
         ms, wall, nonfinite, mass = time_case(sp, args.steps, args.warmup, hip)
@@ -142,7 +174,8 @@ def main():
         results[name] = dict(mlups=round(mlups, 1), ms_per_step=round(ms / args.steps, 4),
                              achieved_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
                              wall_s=round(wall, 3), nonfinite=nonfinite, mass=mass,
-                             bytes_per_site=2 * 19 * sbytes)
+                             bytes_per_site=2 * 19 * sbytes,
+                             streaming="aa" if streaming == abi.STREAM_AA else "two-array")
         print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)",
               file=sys.stderr, flush=True)
     head = results.get("f32") or next(iter(results.values()))
@@ -164,13 +197,13 @@ def main():
         "data": "synthetic (seeded Bernoulli(0.3) solid mask, rest-state start)",
         "config": {"workload": f"D3Q19 TRT porous channel {n}^3, Bernoulli(0.3) solids seed 2, "
                                f"periodic x, no-slip y/z, force 1e-6 x, magic 3/16, tau 0.8, "
-                               f"{head_name} populations, two-array fused pull-stream kernel",
+                               f"{head_name} populations, in-place AA-pattern fused kernel",
                    "lattice": "D3Q19", "nodes": N, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["hbm_frac"], "traffic": None,
                      "algorithmic_bytes_per_launch": bytes_per_launch},
         "cpu_baseline": cpu,
-        "extra": results,
+        "extra": dict(results, copy_gbs=round(copy_gbs, 1)),
     }
     print(json.dumps(line))
 

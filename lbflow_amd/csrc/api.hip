@@ -82,6 +82,21 @@ bool vs_info(int id, VsInfo* v) {
   return false;
 }
 
+// The AA scheme redirects links across non-periodic faces to the node's own
+// slots, which reproduces no-slip and moving-wall bounce-back exactly; it
+// needs opposite faces to be both periodic or both not, and no scalars.
+bool aa_compatible(const lbf_case_desc* d, const VsInfo& v) {
+  if (d->n_scalars) return false;
+  for (int f = 0; f < 2 * v.d; f++) {
+    const int k = d->faces[f].kind;
+    if (k != LBF_BC_NO_SLIP && k != LBF_BC_BOUNCE_BACK && k != LBF_BC_PERIODIC) return false;
+  }
+  for (int a = 0; a < v.d; a++)
+    if ((d->faces[2 * a].kind == LBF_BC_PERIODIC) != (d->faces[2 * a + 1].kind == LBF_BC_PERIODIC))
+      return false;
+  return true;
+}
+
 bool get_launchers(int vs, int prec, Launchers* L) {
   switch (vs) {
     case LBF_D2Q9: return launchers_d2q9(prec, L);
@@ -90,6 +105,41 @@ bool get_launchers(int vs, int prec, Launchers* L) {
     case LBF_D3Q27: return launchers_d3q27(prec, L);
   }
   return false;
+}
+
+// lattice weight from the squared length of c (velocity-set tables:
+// d2q9.rs:20-30, d3q15.rs:25-41, d3q19.rs:29-49, d3q27.rs:37-65)
+double lattice_weight(int vs, const int* c) {
+  const int c2 = c[0] * c[0] + c[1] * c[1] + c[2] * c[2];
+  switch (vs) {
+    case LBF_D2Q9: return c2 == 0 ? 4.0 / 9.0 : (c2 == 1 ? 1.0 / 9.0 : 1.0 / 36.0);
+    case LBF_D3Q15: return c2 == 0 ? 2.0 / 9.0 : (c2 == 1 ? 1.0 / 9.0 : 1.0 / 72.0);
+    case LBF_D3Q19: return c2 == 0 ? 1.0 / 3.0 : (c2 == 1 ? 1.0 / 18.0 : 1.0 / 36.0);
+    case LBF_D3Q27:
+      return c2 == 0 ? 8.0 / 27.0 : (c2 == 1 ? 2.0 / 27.0 : (c2 == 2 ? 1.0 / 54.0 : 1.0 / 216.0));
+  }
+  return 0.0;
+}
+
+// float -> bfloat16 bits, round to nearest even
+uint16_t to_bf16(float f) {
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+void store_shifted(int prec, char* buf, size_t k, double v) {
+  if (prec == LBF_F32_SHIFTED) {
+    reinterpret_cast<float*>(buf)[k] = (float)v;
+  } else if (prec == LBF_F16_SHIFTED) {
+    const _Float16 h = (_Float16)(float)v;
+    memcpy(buf + 2 * k, &h, 2);
+  } else {
+    const uint16_t b = to_bf16((float)v);
+    memcpy(buf + 2 * k, &b, 2);
+  }
 }
 
 size_t storage_bytes(int prec) {
@@ -134,6 +184,12 @@ struct lbf_case {
   double* red_out = nullptr;
   ScalarDev sc[LBF_MAX_SCALARS];
   int ns = 0;
+  bool aa = false;       // in-place AA scheme (one population array)
+  uint64_t host_step = 0;  // AA: steps enqueued (device count, resynchronised on reads)
+  bool aa_fresh = false;   // AA: array holds post-stream f (after init / set_pops)
+  bool may_have_stopped = false;  // steps were enqueued with the stop rule on
+  double scale = 1.0;    // shifted storage: stored = (f - w) * scale
+  double w[27] = {};
   bool frozen = false;
   bool needs_moments = false;  // some condition reads stored moments every step
   uint32_t face_active = 0;
@@ -226,6 +282,7 @@ int sync_and_check(lbf_case* c) {
 int read_ctrl(lbf_case* c, Ctrl* h) {
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  c->host_step = h->step;
   return 0;
 }
 
@@ -288,14 +345,17 @@ int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
   const long long N = (long long)d->n[0] * d->n[1] * (v.d == 3 ? d->n[2] : 1);
   const long long ld = (N + 63) / 64 * 64;
   const size_t sb = storage_bytes(d->precision), tb = d->precision == LBF_F64 ? 8 : 4;
-  uint64_t b = 2ull * v.q * ld * sb + 4ull * ld + 2ull * (1 + v.d) * ld * tb;
+  const bool aa_ok = aa_compatible(d, v);
+  const bool aa = d->precision >= LBF_F32_SHIFTED || d->streaming == LBF_STREAM_AA ||
+                  (d->streaming == LBF_STREAM_AUTO && aa_ok && d->precision != LBF_F64);
+  uint64_t b = (aa ? 1ull : 2ull) * v.q * ld * sb + 4ull * ld + 2ull * (1 + v.d) * ld * tb;
   if (d->force_kind == LBF_SOURCE_FIELD) b += 8ull * v.d * ld;
   for (uint32_t s = 0; s < d->n_scalars; s++) {
     b += 2ull * v.q * ld * tb + 2ull * ld * tb;
     if (d->scalars[s].has_adsorption) b += 8ull * ld;
     if (d->scalars[s].source_kind == LBF_SOURCE_FIELD) b += 8ull * ld;
   }
-  *bytes = b + sizeof(Ctrl) + 2048ull * kMaxCols * 8;
+  *bytes = b + sizeof(Ctrl) + (uint64_t)((N + kBlock - 1) / kBlock) * kMaxCols * 8;
   return 0;
 }
 
@@ -316,6 +376,36 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   c->sb = storage_bytes(c->prec);
   c->tb = c->prec == LBF_F64 ? 8 : 4;
   c->frozen = (d->flags & LBF_CASE_FREEZE_MOMENTUM) != 0;
+  for (int i = 0; i < c->q; i++) c->w[i] = lattice_weight(d->velocity_set, v.c[i]);
+  {
+    const bool shifted = c->prec >= LBF_F32_SHIFTED;
+    const bool aa_ok = aa_compatible(d, v);
+    if (d->streaming == LBF_STREAM_AA && !aa_ok) {
+      delete c;
+      return fail(LBF_EUNSUPPORTED, "AA streaming needs no scalars, face conditions in "
+                                    "{no-slip, bounce-back, periodic} and matching periodicity "
+                                    "of opposite faces");
+    }
+    if (shifted && !aa_ok) {
+      delete c;
+      return fail(LBF_EUNSUPPORTED, "shifted storage runs on the AA kernel only (no scalars, "
+                                    "faces in {no-slip, bounce-back, periodic})");
+    }
+    if (d->streaming == LBF_STREAM_AA) c->aa = true;
+    else if (d->streaming == LBF_STREAM_AUTO) c->aa = aa_ok && c->prec != LBF_F64;
+    if (shifted) c->aa = true;
+    const long long n_nodes = (long long)c->nx * c->ny * c->nz;
+    const long long ld_nodes = (n_nodes + 63) / 64 * 64;
+    if (c->aa && (unsigned long long)ld_nodes * c->q * c->sb >= (1ull << 32)) {
+      if (shifted || d->streaming == LBF_STREAM_AA) {
+        delete c;
+        return fail(LBF_EUNSUPPORTED, "AA population array must stay below 4 GiB");
+      }
+      c->aa = false;
+    }
+    if (shifted) c->scale = d->shift_scale > 0 ? d->shift_scale
+                                               : (c->prec == LBF_F16_SHIFTED ? 4096.0 : 1.0);
+  }
   c->ns = (int)d->n_scalars;
   if (!get_launchers(d->velocity_set, c->prec, &c->L)) {
     delete c;
@@ -380,7 +470,9 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   if (hipGetDeviceProperties(&prop, c->device) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipGetDeviceProperties failed"));
   const long long blocks = (N + kBlock - 1) / kBlock;
-  c->grid = (int)std::max(1LL, std::min<long long>(blocks, (long long)prop.multiProcessorCount * 8));
+  // one node per thread: the streaming kernels have no loop-carried state and
+  // every workgroup writes its own residual partials
+  c->grid = (int)std::max(1LL, blocks);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
   auto alloc = [&](void** p, size_t bytes) -> int {
@@ -391,7 +483,7 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   };
   const size_t pop_bytes = (size_t)q * c->ld * c->sb;
   for (int b = 0; b < 2; b++) {
-    if ((rc = alloc(&c->pop[b], pop_bytes))) return cleanup(rc);
+    if ((b == 0 || !c->aa) && (rc = alloc(&c->pop[b], pop_bytes))) return cleanup(rc);
     if ((rc = alloc(&c->rho[b], c->ld * c->tb))) return cleanup(rc);
     if ((rc = alloc(&c->u[b], (size_t)c->d * c->ld * c->tb))) return cleanup(rc);
   }
@@ -506,6 +598,15 @@ KArgs momentum_args(lbf_case* c) {
   a.force_kind = c->desc.force_kind;
   for (int x = 0; x < 3; x++) a.F[x] = c->desc.force[x];
   a.src_field = c->force;
+  a.scale = c->scale;
+  a.plane_bytes = (uint32_t)(c->ld * c->sb);
+  a.fresh = c->aa_fresh ? 1 : 0;
+  a.aa_odd = (int)(c->host_step & 1);
+  a.inv_scale = 1.0 / c->scale;
+  a.faces_noslip_only = 1;
+  for (int f = 0; f < 2 * c->d; f++)
+    if (c->desc.faces[f].kind != LBF_BC_NO_SLIP && c->desc.faces[f].kind != LBF_BC_PERIODIC)
+      a.faces_noslip_only = 0;
   return a;
 }
 
@@ -557,28 +658,30 @@ int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
   if (!c->frozen) {
     KArgs a = momentum_args(c);
     a.mode = kModePull;
-    a.flags = kf;
+    a.flags = kf & ~kFlagStopCheck;
     if (resid || last_of_batch || c->needs_moments) a.flags |= kFlagStoreMoments;
-    if (c->ns == 0) a.flags |= kFlagAdvance;
-    else a.flags &= ~kFlagStopCheck;
     a.res_mask = mask;
-    HIPCHK(c->L.mom_step(a, c->grid, c->stream));
+    if (c->aa)
+      HIPCHK(c->L.aa_step(a, c->grid, c->stream));
+    else
+      HIPCHK(c->L.mom_step(a, c->grid, c->stream));
   }
   for (int s = 0; s < c->ns; s++) {
     KArgs a = scalar_args(c, s);
     a.mode = kModePull;
-    a.flags = kf;
-    if (s == c->ns - 1) a.flags |= kFlagAdvance;
-    else a.flags &= ~kFlagStopCheck;
+    a.flags = kf & ~kFlagStopCheck;
     a.res_mask = mask;
     HIPCHK(c->L.sca_step(a, c->grid, c->stream));
   }
-  if (c->frozen && c->ns == 0) {
+  {
+    // one-workgroup launch: residual reduction, stop rule, step counter
     KArgs a = momentum_args(c);
-    a.flags = kf | kFlagAdvance;
-    a.res_mask = 0;
-    HIPCHK(launch_advance_only(a, c->stream));
+    a.flags = kf;
+    a.res_mask = mask;
+    HIPCHK(launch_advance(a, c->grid, c->stream));
   }
+  c->host_step++;
+  if (!c->frozen) c->aa_fresh = false;
   return 0;
 }
 
@@ -589,7 +692,11 @@ extern "C" {
 int lbf_initialize_nodes(lbf_case* c) {
   if (!c) return fail(LBF_EINVAL, "case is NULL");
   KArgs a = momentum_args(c);
-  HIPCHK(c->L.mom_init(a, c->grid, c->stream));
+  if (c->aa)
+    HIPCHK(c->L.aa_init(a, c->grid, c->stream));
+  else
+    HIPCHK(c->L.mom_init(a, c->grid, c->stream));
+  c->aa_fresh = true;
   long long fresh = 1;
   for (int s = 0; s < c->ns; s++) {
     KArgs b = scalar_args(c, s);
@@ -602,9 +709,18 @@ int lbf_initialize_nodes(lbf_case* c) {
 
 int lbf_main_steps(lbf_case* c, uint32_t nsteps, uint32_t flags) {
   if (!c) return fail(LBF_EINVAL, "case is NULL");
+  if (c->may_have_stopped) {
+    // a previous call could have stopped early: take the step count from the device
+    Ctrl h;
+    int rc = read_ctrl(c, &h);
+    if (rc) return rc;
+    c->may_have_stopped = false;
+  }
   HIPCHK(launch_ctrl_set(c->ctrl, kCtrlResetStop, 0, c->stream));
-  if (flags & LBF_STEP_STOP_ON_CONVERGENCE)
+  if (flags & LBF_STEP_STOP_ON_CONVERGENCE) {
     HIPCHK(launch_ctrl_set(c->ctrl, kCtrlMaxIter, -1, c->stream));
+    c->may_have_stopped = true;
+  }
   for (uint32_t k = 0; k < nsteps; k++) {
     int rc = enqueue_step(c, flags, k + 1 == nsteps);
     if (rc) return rc;
@@ -756,13 +872,20 @@ int lbf_get_pops(lbf_case* c, int32_t field, double* out) {
   if (field < 0 || field > c->ns || (field == 0 && c->frozen && false))
     return fail(LBF_EINVAL, "no field %d", field);
   const size_t bytes = (size_t)c->q * c->ld * 8;
+  {
+    Ctrl h;  // synchronises and resynchronises the host step count
+    int rc = read_ctrl(c, &h);
+    if (rc) return rc;
+  }
   if (!c->readback) HIPCHK(hipMalloc((void**)&c->readback, bytes));
   HIPCHK(hipMemsetAsync(c->readback, 0, bytes, c->stream));
   KArgs a = field == 0 ? momentum_args(c) : scalar_args(c, field - 1);
   a.mode = kModeStreamOnly;
   a.flags = 0;
   a.out = c->readback;
-  if (field == 0)
+  if (field == 0 && c->aa)
+    HIPCHK(c->L.aa_readback(a, c->grid, c->stream));
+  else if (field == 0)
     HIPCHK(c->L.mom_step(a, c->grid, c->stream));
   else
     HIPCHK(c->L.sca_step(a, c->grid, c->stream));
@@ -784,19 +907,25 @@ int lbf_set_pops(lbf_case* c, int32_t field, const double* in) {
   std::vector<double> soa((size_t)c->q * c->ld, 0.0);
   for (long long i = 0; i < c->N; i++)
     for (int j = 0; j < c->q; j++) soa[(size_t)j * c->ld + i] = in[i * c->q + j];
-  void* dst = field == 0 ? c->pop[par] : c->sc[field - 1].pop[par];
+  void* dst = field == 0 ? c->pop[c->aa ? 0 : par] : c->sc[field - 1].pop[par];
   const size_t elem = field == 0 ? c->sb : c->tb;
   std::vector<char> buf(soa.size() * elem);
-  if (elem == 8) {
+  if (field == 0 && c->prec >= LBF_F32_SHIFTED) {
+    for (int j = 0; j < c->q; j++)
+      for (long long i = 0; i < c->ld; i++) {
+        const size_t k = (size_t)j * c->ld + i;
+        const double dv = i < c->N ? (soa[k] - c->w[j]) * c->scale : 0.0;
+        store_shifted(c->prec, buf.data(), k, dv);
+      }
+  } else if (elem == 8) {
     memcpy(buf.data(), soa.data(), buf.size());
-  } else if (field != 0 || c->prec == LBF_F32) {
+  } else {
     float* o = reinterpret_cast<float*>(buf.data());
     for (size_t i = 0; i < soa.size(); i++) o[i] = (float)soa[i];
-  } else {
-    return fail(LBF_EUNSUPPORTED, "set_pops for shifted storage");
   }
   HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
   HIPCHK(launch_ctrl_set(c->ctrl, kCtrlFresh, 1ll << field, c->stream));
+  if (field == 0) c->aa_fresh = true;
   return 0;
 }
 

@@ -13,6 +13,9 @@ struct Launchers {
   LaunchFn mom_init = nullptr;  // f = f_eq
   LaunchFn sca_step = nullptr;
   LaunchFn sca_init = nullptr;
+  LaunchFn aa_step = nullptr;   // in-place AA-pattern step (momentum only)
+  LaunchFn aa_init = nullptr;
+  LaunchFn aa_readback = nullptr;
 };
 
 // precision: LBF_F64 .. LBF_BF16_SHIFTED; returns false when not built
@@ -23,7 +26,9 @@ bool launchers_d3q27(int precision, Launchers* out);
 
 // lattice-independent kernels (misc.hip)
 hipError_t launch_ctrl_set(Ctrl* ctrl, int what, long long value, hipStream_t s);
-hipError_t launch_advance_only(const KArgs& a, hipStream_t s);
+// one-workgroup launch that ends a step (residual reduction, stop rule,
+// step counter); nblocks = workgroups of the step's field launches
+hipError_t launch_advance(const KArgs& a, long long nblocks, hipStream_t s);
 hipError_t launch_reduce(const void* values, int is_double, const uint32_t* cls, int need_link,
                          long long N, double* partials, int grid, double* out, int count_nonfinite,
                          hipStream_t s);

@@ -17,7 +17,6 @@ enum Mode : int { kModePull = 0, kModeStreamOnly = 1 };
 enum Flag : uint32_t {
   kFlagStoreMoments = 1u,
   kFlagResiduals = 2u,
-  kFlagAdvance = 4u,     // this launch ends the step: advance the device step counter
   kFlagStopCheck = 8u,   // evaluate the reference stop rule after the step
 };
 
@@ -27,7 +26,7 @@ struct Ctrl {
   unsigned long long max_iter; // stop when step index > max_iter (with kFlagStopCheck)
   int stopped;
   int fresh[5];                // field holds post-stream f (after init / set_pops)
-  unsigned done;               // workgroups finished in the current advancing launch
+  unsigned done;               // reserved
   unsigned pad;
   double res[kMaxCols];        // residual norms of the last step
   double hist[kHist][kMaxCols];
@@ -57,6 +56,10 @@ struct KArgs {
   int field;          // 0 momentum, 1+s scalar
   int nfaces;
   uint32_t face_active;  // faces with a non-periodic condition
+  int faces_noslip_only; // every non-periodic face is no-slip (AA: nothing to apply)
+  int fresh;             // AA: the array holds post-stream f in natural slots
+  int aa_odd;            // AA: parity of the step being launched
+  uint32_t plane_bytes;  // AA: bytes per population plane (array < 4 GiB)
   void* pop[2];          // ping-pong population buffers (two-array scheme)
   void* out;             // stream-only destination
   const uint32_t* cls;

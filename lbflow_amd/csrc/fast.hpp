@@ -1,0 +1,427 @@
+// fast.hpp — in-place AA-pattern step kernels for the throughput tiers.
+//
+// One population array F[q][ld]. Even steps read the node's own slots
+// F[j][x] (= post-stream f_j), collide, and write f*_j to the swapped own slot
+// F[jbar][x]. Odd steps read f_j(x) from F[jbar][x - c_j] (the neighbour's
+// swapped slot), collide, and write f*_j to F[j][x + c_j]. Every slot is read
+// and written by the same thread within a step, so the update is in place:
+// half the memory of a two-array scheme at the same 2*Q*s bytes per update.
+//
+// Links whose neighbour is solid, or that cross a face with a non-periodic
+// condition, are redirected to the node's own slots (the value that would be
+// pulled back is the node's own f*_i, i.e. bounce-back), so the post-stream
+// population arriving through such a link is f_ibar = f*_i; face conditions
+// are then applied in face order from those values (src/momentum/bc.rs:
+// 29-129). This requires opposite faces to be both periodic or both not,
+// which the host checks before choosing this scheme.
+//
+// Addressing: one buffer resource over the whole population array, a 32-bit
+// per-thread byte offset (shared by all planes of a neighbour) and a
+// per-plane scalar offset, so each load/store needs one VGPR of address. The
+// host uses this scheme only while the array is below 4 GiB.
+//
+// Storage: plain f (f64 / f32) or shifted delta_i = (f_i - w_i) * scale in
+// f32, fp16 or bf16 with all arithmetic in f32 on delta (node_math.hpp).
+#pragma once
+
+#include "engine.hpp"
+#include "kernels.hpp"
+#include "node_math.hpp"
+
+namespace lbf {
+
+// equilibrium moments minus M w, in delta form, for shifted MRT:
+// meq_k(rho,u) - (M w)_k = drho g_k(0) + rho h_k(u) + (g_k(0) - (M w)_k)
+// with meq_k(rho, u) = rho g_k(u) the reference's closed forms and
+// h_k(u) = g_k(u) - g_k(0) written out so no constant cancels in f32.
+template <class L>
+__device__ __forceinline__ void meq_shifted(float drho, float rho, const float* u,
+                                            const float* deq, float* m) {
+  using Dv = Derived<L>;
+  auto mw = [](int k) {
+    double s = 0;
+    for (int i = 0; i < L::Q; i++) s += Dv::M(k, i) * Dv::w(i);
+    return s;
+  };
+  if constexpr (L::id == 1) {
+    const float ux = u[0], uy = u[1], u2 = ux * ux + uy * uy;
+    const float h[9] = {0.f, -3.f * u2, 9.f * ux * ux * uy * uy - 3.f * u2, ux,
+                        3.f * ux * ux * ux - ux, uy, 3.f * uy * uy * uy - uy, u2, ux * uy};
+    constexpr double g0[9] = {1, 1, 1, 0, 0, 0, 0, 0, 0};
+    sfor<9>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr float c = float(g0[k] - mw(k));
+      m[k] = drho * float(g0[k]) + rho * h[k] + c;
+    });
+  } else if constexpr (L::id == 3) {
+    const float ux = u[0], uy = u[1], uz = u[2], u2 = ux * ux + uy * uy + uz * uz;
+    const float h9 = 2.f * ux * ux - (uy * uy + uz + uz);
+    const float h11 = uy * uy - uz * uz;
+    const float h[19] = {0.f, 19.f * u2, -5.5f * u2, ux, (-2.f / 3.f) * ux, uy, (-2.f / 3.f) * uy,
+                         uz, (-2.f / 3.f) * uz, h9, -0.5f * h9, h11, -0.5f * h11, ux * uy,
+                         uy * uz, ux * uz, 0.f, 0.f, 0.f};
+    constexpr double g0[19] = {1, -11, 3, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    sfor<19>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr float c = float(g0[k] - mw(k));
+      m[k] = drho * float(g0[k]) + rho * h[k] + c;
+    });
+  } else if constexpr (L::id == 2) {
+    const float ux = u[0], uy = u[1], uz = u[2], u2 = ux * ux + uy * uy + uz * uz;
+    const float h[15] = {0.f, u2, -5.f * u2, ux, (-7.f / 3.f) * ux, uy, (-7.f / 3.f) * uy, uz,
+                         (-7.f / 3.f) * uz, 2.f * ux * ux - (uy * uy + uz * uz), uy * uy - uz * uz,
+                         ux * uy, uy * uz, ux * uz, 0.f};
+    constexpr double g0[15] = {1, -1, 1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    sfor<15>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      constexpr float c = float(g0[k] - mw(k));
+      m[k] = drho * float(g0[k]) + rho * h[k] + c;
+    });
+  } else {
+    // no closed form: m_eq = M f_eq, so m_eq - M w = M delta_eq
+    sfor<L::Q>([&](auto K) {
+      constexpr int k = decltype(K)::value;
+      m[k] = mrt_dot<L, k>(deq);
+    });
+  }
+}
+
+// ------------------------------------------------------------ buffer I/O
+template <class S>
+__device__ __forceinline__ S bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(S) == 8)
+    return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else if constexpr (sizeof(S) == 4)
+    return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0));
+}
+template <class S>
+__device__ __forceinline__ void bstore(S v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  if constexpr (sizeof(S) == 8) {
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, voff, soff, 0);
+  }
+  else if constexpr (sizeof(S) == 4)
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+  else
+    __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
+}
+
+// directions crossing face F (c . n_out = +1)
+template <class L>
+constexpr uint32_t face_dirs(int F) {
+  uint32_t m = 0;
+  for (int i = 0; i < L::Q; i++)
+    if (Derived<L>::in_face(F, i)) m |= 1u << i;
+  return m;
+}
+
+struct NodePos {
+  uint32_t on;   // faces (with a non-periodic condition) the node lies on
+  int32_t ox[3], oy[3], oz[3];  // wrapped neighbour offsets for shifts -1, 0, +1
+};
+
+template <class L>
+__device__ __forceinline__ void node_pos(const KArgs& a, uint32_t n, bool need_offsets,
+                                         NodePos& p) {
+  const uint32_t nx = (uint32_t)a.nx, ny = (uint32_t)a.ny, nz = (uint32_t)a.nz;
+  const uint32_t nxy = nx * ny;
+  const uint32_t z = n / nxy;
+  const uint32_t r = n - z * nxy;
+  const uint32_t y = r / nx;
+  const uint32_t x = r - y * nx;
+  uint32_t on = 0;
+  on |= (x == 0) ? 1u : 0u;
+  on |= (x == nx - 1) ? 2u : 0u;
+  on |= (y == 0) ? 4u : 0u;
+  on |= (y == ny - 1) ? 8u : 0u;
+  if constexpr (L::D == 3) {
+    on |= (z == 0) ? 16u : 0u;
+    on |= (z == nz - 1) ? 32u : 0u;
+  }
+  p.on = on & a.face_active;
+  if (need_offsets) {
+    p.ox[0] = x == 0 ? (int32_t)(nx - 1) : -1;
+    p.ox[1] = 0;
+    p.ox[2] = x == nx - 1 ? -(int32_t)(nx - 1) : 1;
+    p.oy[0] = y == 0 ? (int32_t)((ny - 1) * nx) : -(int32_t)nx;
+    p.oy[1] = 0;
+    p.oy[2] = y == ny - 1 ? -(int32_t)((ny - 1) * nx) : (int32_t)nx;
+    p.oz[0] = z == 0 ? (int32_t)((nz - 1) * nxy) : -(int32_t)nxy;
+    p.oz[1] = 0;
+    p.oz[2] = z == nz - 1 ? -(int32_t)((nz - 1) * nxy) : (int32_t)nxy;
+  }
+}
+
+template <class L>
+__device__ __forceinline__ uint32_t redirect_mask(uint32_t cls, uint32_t on) {
+  uint32_t red = cls & kLinkMask;
+  if (on) {
+    sfor<2 * L::D>([&](auto FI) {
+      constexpr int Fc = decltype(FI)::value;
+      if (on >> Fc & 1u) red |= face_dirs<L>(Fc);
+    });
+  }
+  return red;
+}
+
+// Post-stream populations of node n (compute type, delta form when shifted).
+template <class L, class St, bool ODD>
+__device__ __forceinline__ void aa_gather(__amdgpu_buffer_rsrc_t rs, uint32_t plane_bytes,
+                                          uint32_t n, bool fresh, uint32_t red, const NodePos& p,
+                                          typename St::T* f) {
+  using S = typename St::S;
+  using T = typename St::T;
+  using k = K<L>;
+  const uint32_t own = n * (uint32_t)sizeof(S);
+  sfor<L::Q>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    constexpr int jb = k::qbar(j);
+    S v;
+    if (!ODD || fresh) {
+      v = bload<S>(rs, own, j * plane_bytes);
+    } else {
+      // one load per direction: the redirect only selects the lane's address
+      const int32_t d = p.ox[1 - k::cx(j)] + p.oy[1 - k::cy(j)] + p.oz[1 - k::cz(j)];
+      const uint32_t nb = (uint32_t)((int32_t)n + d) * (uint32_t)sizeof(S) + jb * plane_bytes;
+      const uint32_t mine = own + j * plane_bytes;
+      v = bload<S>(rs, (red >> jb & 1u) ? mine : nb, 0u);
+    }
+    f[j] = T(v);  // shifted storage: scaled by the caller
+  });
+}
+
+// face conditions from the redirected values (f_ibar = f*_i on crossing links)
+template <class L, class T>
+__device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
+  using k = K<L>;
+  using Dv = Derived<L>;
+  T fsv[L::Q];
+  sfor<L::Q>([&](auto J) { fsv[decltype(J)::value] = f[decltype(J)::value]; });
+  sfor<2 * L::D>([&](auto FI) {
+    constexpr int Fc = decltype(FI)::value;
+    if (!(on >> Fc & 1u)) return;
+    const FaceK& bc = a.face[Fc];
+    T vel[3] = {T(0), T(0), T(0)};
+    if (bc.kind == 2)
+      sfor<L::D>([&](auto A) { vel[decltype(A)::value] = T(bc.vel[decltype(A)::value]); });
+    sfor<L::NQF>([&](auto KI) {
+      constexpr int i = Dv::qface(Fc, decltype(KI)::value);
+      constexpr int ib = k::qbar(i);
+      if (bc.kind == 1) {
+        f[ib] = fsv[ib];
+      } else if (bc.kind == 2) {
+        const T udc = udotc<L, i>(vel);
+        f[ib] = fsv[ib] - T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+      }
+    });
+  });
+}
+
+// AA-pattern fused step; OP 0 BGK, 1 TRT, 2 MRT; ODD selects the step parity.
+// MOM: this launch may store moments / residual partials (the last step of a
+// batch, or steps with residuals); the streaming steps in between use the
+// MOM=false instantiation, which is compiled for full occupancy.
+template <class L, class St, int OP, bool ODD, bool MOM>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(MOM ? 2 : (sizeof(typename St::T) == 8 ? 4 : (ODD ? 6 : 8)))))
+    momentum_step_aa(KArgs a) {
+#pragma clang fp contract(fast)
+  using T = typename St::T;
+  using S = typename St::S;
+  using k = K<L>;
+  constexpr int Q = L::Q, D = L::D;
+  Ctrl* ctrl = a.ctrl;
+  if (ctrl->stopped) return;
+  const unsigned long long step = ctrl->step;
+  const bool fresh = a.fresh != 0;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(a.pop[0], (short)0, (int)0xffffffff, 0x00020000);
+  const uint32_t pb = a.plane_bytes;
+  const int par = (int)(step & 1);
+  T* rho_c = (T*)a.rho[par];
+  T* u_c = (T*)a.u[par];
+  const T* rho_p = (const T*)a.rho[par ^ 1];
+  const T* u_p = (const T*)a.u[par ^ 1];
+  const bool store_m = MOM && (a.flags & (kFlagStoreMoments | kFlagResiduals)) != 0;
+  const bool resid = MOM && (a.flags & kFlagResiduals) != 0;
+  const T omega = T(a.omega), omega_p = T(a.omega_p), wp = T(a.wp), wm = T(a.wm),
+          coeff_b = T(a.coeff_b);
+  const T sc = T(a.scale), isc = T(a.inv_scale);
+  const uint32_t N = (uint32_t)a.N;
+  const uint32_t ld = (uint32_t)a.ld;
+  const bool any_face = a.face_active != 0;
+  double acc[5] = {0, 0, 0, 0, 0};
+  // one node per thread (the host launches ceil(N / kBlock) workgroups)
+  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t cls = n < N ? a.cls[n] : kSolidBit;
+  if (!(cls & kSolidBit)) {
+    NodePos p;
+    p.on = 0;
+    if (ODD || any_face) node_pos<L>(a, n, ODD, p);
+    const uint32_t red = ODD ? redirect_mask<L>(cls, p.on) : 0u;
+    T f[Q];
+    aa_gather<L, St, ODD>(rs, pb, n, fresh, red, p, f);
+    if constexpr (St::shifted)
+      sfor<Q>([&](auto J) { f[decltype(J)::value] *= isc; });
+    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T>(a, p.on, f);
+    // ---- moments
+    // (this tier is not held to the reference's summation order)
+    T rho, u[3], drho = f[0];
+    sfor<Q - 1>([&](auto J) { drho += f[decltype(J)::value + 1]; });
+    rho = St::shifted ? T(1) + drho : drho;
+    sfor<D>([&](auto A) {
+      constexpr int ax = decltype(A)::value;
+      T s = T(0);
+      sfor<Q>([&](auto J) {
+        constexpr int j = decltype(J)::value;
+        constexpr int c = k::ca(j, ax);
+        if constexpr (c > 0) s += f[j];
+        if constexpr (c < 0) s -= f[j];
+      });
+      u[ax] = s / rho;
+    });
+    T Fv[3] = {T(0), T(0), T(0)};
+    if (a.force_kind) {
+      sfor<D>([&](auto A) {
+        constexpr int ax = decltype(A)::value;
+        Fv[ax] = a.force_kind == 1 ? T(a.F[ax]) : T(a.src_field[(size_t)ax * ld + n]);
+        u[ax] += T(0.5) * Fv[ax] / rho;
+      });
+    }
+    if (store_m) {
+      rho_c[n] = rho;
+      sfor<D>([&](auto A) { u_c[(size_t)decltype(A)::value * ld + n] = u[decltype(A)::value]; });
+      if (resid) {
+        const double dr = (double)rho - (double)rho_p[n];
+        acc[0] += dr * dr;
+        bool bad = !isfinite((double)rho);
+        sfor<D>([&](auto A) {
+          constexpr int ax = decltype(A)::value;
+          const double du = (double)u[ax] - (double)u_p[(size_t)ax * ld + n];
+          acc[1 + ax] += du * du;
+          bad = bad || !isfinite((double)u[ax]);
+        });
+        acc[4] += bad ? 1.0 : 0.0;
+      }
+    }
+    // ---- collision
+    T feq[Q], fs[Q];
+    if constexpr (St::shifted)
+      equilibrium_shifted<L>(drho, rho, u, feq);
+    else
+      equilibrium<L>(rho, u, feq);
+    if constexpr (OP == 0) {
+      bgk<L>(f, feq, omega, omega_p, fs);
+      if (a.force_kind) add_guo<L>(u, Fv, coeff_b, fs);
+    } else if constexpr (OP == 1) {
+      trt<L>(f, feq, wp, wm, fs);
+    } else {
+      T s[Q];
+      sfor<Q>([&](auto KI) { s[decltype(KI)::value] = T(a.s[decltype(KI)::value]); });
+      if constexpr (St::shifted) {
+        T m[Q], meq[Q], ms[Q];
+        sfor<Q>([&](auto Kr) { m[decltype(Kr)::value] = mrt_dot<L, decltype(Kr)::value>(f); });
+        meq_shifted<L>(drho, rho, u, feq, meq);
+        sfor<Q>([&](auto Kr) {
+          constexpr int kk = decltype(Kr)::value;
+          ms[kk] = m[kk] - s[kk] * (m[kk] - meq[kk]);
+        });
+        sfor<Q>([&](auto I) { fs[decltype(I)::value] = mrt_inv_dot<L, decltype(I)::value>(ms); });
+      } else {
+        mrt<L>(rho, u, f, feq, s, fs);
+      }
+    }
+    // ---- scatter
+    const uint32_t own = n * (uint32_t)sizeof(S);
+    sfor<Q>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      constexpr int jb = k::qbar(j);
+      T v = fs[j];
+      if constexpr (St::shifted) v = v * sc;
+      const S sv = static_cast<S>(v);
+      if (!ODD) {
+        bstore<S>(sv, rs, own, jb * pb);
+      } else {
+        const int32_t d = p.ox[1 + k::cx(j)] + p.oy[1 + k::cy(j)] + p.oz[1 + k::cz(j)];
+        const uint32_t nb = (uint32_t)((int32_t)n + d) * (uint32_t)sizeof(S) + j * pb;
+        const uint32_t mine = own + jb * pb;
+        bstore<S>(sv, rs, (red >> j & 1u) ? mine : nb, 0u);
+      }
+    });
+  }
+  if (a.flags & kFlagResiduals) {
+    __shared__ double lds[kBlock / 64];
+    for (int kk = 0; kk < 4; kk++) {
+      const double s = block_sum(acc[kk], lds);
+      if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + kk] = s;
+    }
+    const double s = block_sum(acc[4], lds);
+    if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + 8] = s;
+  }
+}
+
+// Post-stream populations (f, not delta) into out[q][ld] as f64: the first
+// half of the next step, for read-back.
+template <class L, class St, bool ODD>
+__global__ void __launch_bounds__(kBlock) momentum_readback_aa(KArgs a) {
+  using T = typename St::T;
+  using k = K<L>;
+  constexpr int Q = L::Q;
+  const bool fresh = a.fresh != 0;
+  const __amdgpu_buffer_rsrc_t rs =
+      __builtin_amdgcn_make_buffer_rsrc(a.pop[0], (short)0, (int)0xffffffff, 0x00020000);
+  double* outd = (double*)a.out;
+  const uint32_t N = (uint32_t)a.N;
+  for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < N; n += gridDim.x * kBlock) {
+    const uint32_t cls = a.cls[n];
+    if (cls & kSolidBit) continue;
+    NodePos p;
+    node_pos<L>(a, n, true, p);
+    const uint32_t red = redirect_mask<L>(cls, p.on);
+    T f[Q];
+    aa_gather<L, St, ODD>(rs, a.plane_bytes, n, fresh, red, p, f);
+    if constexpr (St::shifted)
+      sfor<Q>([&](auto J) { f[decltype(J)::value] *= T(a.inv_scale); });
+    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T>(a, p.on, f);
+    sfor<Q>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      outd[(size_t)j * a.ld + n] = St::shifted ? (double)f[j] + k::w(j) : (double)f[j];
+    });
+  }
+}
+
+// f = f_eq (or delta = delta_eq scaled) into the natural slots F[j][x]
+template <class L, class St>
+__global__ void __launch_bounds__(kBlock) momentum_init_aa(KArgs a) {
+  using T = typename St::T;
+  using S = typename St::S;
+  constexpr int Q = L::Q, D = L::D;
+  const int cur = (int)((a.ctrl->step + 1) & 1);
+  S* F = (S*)a.pop[0];
+  const T* rho = (const T*)a.rho[cur];
+  const T* uu = (const T*)a.u[cur];
+  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < a.N;
+       n += (long long)gridDim.x * kBlock) {
+    T feq[Q];
+    if (a.cls[n] & kSolidBit) {
+      sfor<Q>([&](auto J) { feq[decltype(J)::value] = T(0); });
+    } else {
+      T u[3];
+      sfor<D>([&](auto A) { u[decltype(A)::value] = uu[decltype(A)::value * a.ld + n]; });
+      if constexpr (St::shifted)
+        equilibrium_shifted<L>(rho[n] - T(1), rho[n], u, feq);
+      else
+        equilibrium<L>(rho[n], u, feq);
+    }
+    sfor<Q>([&](auto J) {
+      constexpr int j = decltype(J)::value;
+      T v = feq[j];
+      if constexpr (St::shifted) v = v * T(a.scale);
+      F[(size_t)j * a.ld + n] = static_cast<S>(v);
+    });
+  }
+}
+
+}  // namespace lbf

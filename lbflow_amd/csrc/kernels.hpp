@@ -16,11 +16,11 @@
 // post-stream f (after initialize_nodes / set_pops), whose first launch only
 // collides. Populations live in structure-of-arrays planes [q][ld].
 //
-// The launch that ends a step (kFlagAdvance) counts finished workgroups; the
-// last one reduces the per-workgroup residual partials in a fixed order,
-// applies the stop rule (src/momentum/lattice.rs:407-418) and advances the
-// device step counter, so the host never synchronises inside a run and a
-// captured sequence of launches can be replayed.
+// Each step ends with a one-workgroup launch (advance_core) that reduces the
+// per-workgroup residual partials in a fixed order, applies the stop rule
+// (src/momentum/lattice.rs:407-418) and advances the device step counter, so
+// the host never synchronises inside a run and a captured sequence of
+// launches can be replayed.
 #pragma once
 
 #include "engine.hpp"
@@ -124,22 +124,17 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   return r;
 }
 
-// Called by every thread of every workgroup at the end of a launch carrying
-// kFlagAdvance: the last workgroup to finish reduces the residual partials of
-// the step (columns in a.res_mask, summed in workgroup order), applies the
-// stop rule and advances the device step counter.
-__device__ __forceinline__ void advance_step(const KArgs& a, unsigned long long step) {
+// Ends a step; run by the single workgroup of the advance launch that
+// follows the step's field launches (stream order makes their writes
+// visible). Reduces the residual partials of the step's `nblocks`
+// workgroups (columns in a.res_mask, summed in workgroup order), applies
+// the stop rule (src/momentum/lattice.rs:407-418) and advances the device
+// step counter. Keeping this out of the streaming kernels avoids a
+// device-scope release fence per workgroup, which would write back L2
+// lines that other waves have only partly filled.
+__device__ __forceinline__ void advance_core(const KArgs& a, unsigned long long step,
+                                             long long nblocks) {
   __shared__ double lds[kBlock / 64];
-  __shared__ int am_last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(&a.ctrl->done, 1u);
-    am_last = (prev == gridDim.x - 1) ? 1 : 0;
-  }
-  __syncthreads();
-  if (!am_last) return;
-  __threadfence();
   Ctrl* ctrl = a.ctrl;
   double res[kMaxCols];
   for (int k = 0; k < kMaxCols; k++) res[k] = 0.0;
@@ -148,8 +143,7 @@ __device__ __forceinline__ void advance_step(const KArgs& a, unsigned long long 
     for (int k = 0; k < kMaxCols; k++) {
       if (!(a.res_mask >> k & 1u)) continue;
       double s = 0.0;
-      for (long long b = threadIdx.x; b < gridDim.x; b += kBlock)
-        s += __builtin_nontemporal_load(&a.partials[b * kMaxCols + k]);
+      for (long long b = threadIdx.x; b < nblocks; b += kBlock) s += a.partials[b * kMaxCols + k];
       res[k] = block_sum(s, lds);
     }
   }
@@ -170,7 +164,6 @@ __device__ __forceinline__ void advance_step(const KArgs& a, unsigned long long 
     ctrl->step = step + 1;
     for (int f = 0; f < 5; f++) ctrl->fresh[f] = 0;
     if (stop) ctrl->stopped = 1;
-    ctrl->done = 0;
   }
 }
 
@@ -349,7 +342,6 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
     const double s = block_sum(acc[4], lds);
     if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + 8] = s;
   }
-  if (a.flags & kFlagAdvance) advance_step(a, step);
 }
 
 // ------------------------------------------------------------- scalar
@@ -591,7 +583,6 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
     const double s = block_sum(acc, lds);
     if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + 4 + (a.field - 1)] = s;
   }
-  if (a.flags & kFlagAdvance) advance_step(a, step);
 }
 
 // ------------------------------------------------------------- init

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "dispatch.hpp"
+#include "fast.hpp"
 #include "kernels.hpp"
 
 namespace lbf {
@@ -34,21 +35,69 @@ template <class L, class T>
 static hipError_t sca_init_t(const KArgs& a, int grid, hipStream_t s) {
   return launch(scalar_init<L, T>, a, grid, s);
 }
+template <class L, class St, bool ODD>
+static hipError_t aa_step_p(const KArgs& a, int grid, hipStream_t s) {
+  const bool mom = (a.flags & (kFlagStoreMoments | kFlagResiduals)) != 0;
+  if (mom) {
+    if (a.op == 0) return launch(momentum_step_aa<L, St, 0, ODD, true>, a, grid, s);
+    if (a.op == 1) return launch(momentum_step_aa<L, St, 1, ODD, true>, a, grid, s);
+    if constexpr (L::has_mrt) return launch(momentum_step_aa<L, St, 2, ODD, true>, a, grid, s);
+  } else {
+    if (a.op == 0) return launch(momentum_step_aa<L, St, 0, ODD, false>, a, grid, s);
+    if (a.op == 1) return launch(momentum_step_aa<L, St, 1, ODD, false>, a, grid, s);
+    if constexpr (L::has_mrt) return launch(momentum_step_aa<L, St, 2, ODD, false>, a, grid, s);
+  }
+  return hipErrorInvalidValue;
+}
+template <class L, class St>
+static hipError_t aa_step_t(const KArgs& a, int grid, hipStream_t s) {
+  return a.aa_odd ? aa_step_p<L, St, true>(a, grid, s) : aa_step_p<L, St, false>(a, grid, s);
+}
+template <class L, class St>
+static hipError_t aa_readback_t(const KArgs& a, int grid, hipStream_t s) {
+  return a.aa_odd ? launch(momentum_readback_aa<L, St, true>, a, grid, s)
+                  : launch(momentum_readback_aa<L, St, false>, a, grid, s);
+}
+template <class L, class St>
+static hipError_t aa_init_t(const KArgs& a, int grid, hipStream_t s) {
+  return launch(momentum_init_aa<L, St>, a, grid, s);
+}
 
 template <class L>
 static bool make_launchers(int precision, Launchers* out) {
   switch (precision) {
-    case 0:  // LBF_F64
+    case 0:  // LBF_F64: parity path (two arrays) and AA on request
       out->mom_step = mom_step_plain<L, StF64>;
       out->mom_init = mom_init_t<L, StF64>;
       out->sca_step = sca_step_t<L, double>;
       out->sca_init = sca_init_t<L, double>;
+      out->aa_step = aa_step_t<L, StF64>;
+      out->aa_init = aa_init_t<L, StF64>;
+      out->aa_readback = aa_readback_t<L, StF64>;
       return true;
     case 1:  // LBF_F32
       out->mom_step = mom_step_plain<L, StF32>;
       out->mom_init = mom_init_t<L, StF32>;
       out->sca_step = sca_step_t<L, float>;
       out->sca_init = sca_init_t<L, float>;
+      out->aa_step = aa_step_t<L, StF32>;
+      out->aa_init = aa_init_t<L, StF32>;
+      out->aa_readback = aa_readback_t<L, StF32>;
+      return true;
+    case 2:  // LBF_F32_SHIFTED
+      out->aa_step = aa_step_t<L, StF32S>;
+      out->aa_init = aa_init_t<L, StF32S>;
+      out->aa_readback = aa_readback_t<L, StF32S>;
+      return true;
+    case 3:  // LBF_F16_SHIFTED
+      out->aa_step = aa_step_t<L, StF16S>;
+      out->aa_init = aa_init_t<L, StF16S>;
+      out->aa_readback = aa_readback_t<L, StF16S>;
+      return true;
+    case 4:  // LBF_BF16_SHIFTED
+      out->aa_step = aa_step_t<L, StBF16S>;
+      out->aa_init = aa_init_t<L, StBF16S>;
+      out->aa_readback = aa_readback_t<L, StBF16S>;
       return true;
     default:
       return false;

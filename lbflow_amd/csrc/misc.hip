@@ -29,14 +29,14 @@ hipError_t launch_ctrl_set(Ctrl* ctrl, int what, long long value, hipStream_t s)
   return hipGetLastError();
 }
 
-__global__ void __launch_bounds__(kBlock) advance_only_kernel(KArgs a) {
+__global__ void __launch_bounds__(kBlock) advance_kernel(KArgs a, long long nblocks) {
   if (a.ctrl->stopped) return;
   const unsigned long long step = a.ctrl->step;
-  advance_step(a, step);
+  advance_core(a, step, nblocks);
 }
 
-hipError_t launch_advance_only(const KArgs& a, hipStream_t s) {
-  hipLaunchKernelGGL(advance_only_kernel, dim3(1), dim3(kBlock), 0, s, a);
+hipError_t launch_advance(const KArgs& a, long long nblocks, hipStream_t s) {
+  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(kBlock), 0, s, a, nblocks);
   return hipGetLastError();
 }
 
@@ -113,4 +113,22 @@ hipError_t launch_reduce(const void* values, int is_double, const uint32_t* cls,
   return hipGetLastError();
 }
 
+// Plain device copy (16-byte vectors), used to calibrate memory counters:
+// a copy of B bytes moves exactly B bytes in and B bytes out.
+__global__ void __launch_bounds__(kBlock) copy_kernel(const uint4* __restrict__ src,
+                                                      uint4* __restrict__ dst, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * kBlock)
+    dst[i] = src[i];
+}
+
 }  // namespace lbf
+
+// Benchmark hook (not part of the reference-equivalent ABI): copies `bytes`
+// (a multiple of 16) from src to dst on `stream`.
+extern "C" int lbfx_copy(void* dst, const void* src, size_t bytes, void* stream) {
+  const size_t n = bytes / 16;
+  hipLaunchKernelGGL(lbf::copy_kernel, dim3(2048), dim3(lbf::kBlock), 0, (hipStream_t)stream,
+                     (const uint4*)src, (uint4*)dst, n);
+  return hipGetLastError() == hipSuccess ? 0 : -3;
+}

@@ -69,4 +69,9 @@ def test_validation_errors_without_gpu():
     ok = configs.porous_channel(n=16, precision=abi.F32)
     desc, keep = ok.to_c()
     assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == 0
+    # the in-place AA scheme holds one population array
+    assert 19 * 16 ** 3 * 4 <= out.value < 2 * 19 * 16 ** 3 * 4
+    two = configs.porous_channel(n=16, precision=abi.F32, streaming=abi.STREAM_TWO_ARRAY)
+    desc, keep = two.to_c()
+    assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == 0
     assert out.value >= 2 * 19 * 16 ** 3 * 4

@@ -1,0 +1,114 @@
+"""GPU checks of the throughput tiers: the in-place AA kernel and shifted
+storage (f - w) in FP32 / FP16 / BF16 with FP32 arithmetic.
+
+Tolerances (stated per tier, against the f64 oracle on the same input):
+  AA f64             1e-12 relative on populations and moments (the AA
+                     kernel allows fused multiply-add, so not bit-exact)
+  AA f32 / f32s      1e-5 relative on populations, 1e-4 relative L2 on u
+  f16s               1e-3 relative L2 velocity error (the precision target)
+  bf16s              reported, bounded by 3e-2 relative L2 velocity error
+Mass drift of every tier is checked against the oracle's total mass.
+"""
+import numpy as np
+import pytest
+
+from lbflow_amd import abi, configs
+from lbflow_amd.case import Collision
+from lbflow_amd.engine import Case
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def l2rel(a, b):
+    return np.linalg.norm(np.asarray(a) - b) / max(np.linalg.norm(b), 1e-300)
+
+
+AA_CASES = {
+    "porous_trt_force": configs.porous_channel(n=(24, 20, 16), force=1e-4),
+    "porous_bgk_guo": configs.porous_channel(n=(16, 12, 20), op=Collision.bgk(0.7), force=1e-4, seed=5),
+    "cavity_moving_lid": configs.cavity(n=24, tau=0.6),
+    "periodic_mrt_d3q19": configs.porous_channel(n=(12, 10, 8), force=1e-4, op=Collision.mrt(
+        [1.0, 1.19, 1.4, 1.0, 1.2, 1.0, 1.2, 1.0, 1.2, 1.25, 1.4, 1.25, 1.4, 1.25, 1.25, 1.25, 1.98, 1.98, 1.98])),
+    "d3q27_bgk": configs.porous_channel(n=(10, 8, 9), velocity_set=abi.D3Q27, op=Collision.bgk(0.8), force=1e-4),
+    "d3q15_trt": configs.porous_channel(n=(10, 9, 8), velocity_set=abi.D3Q15, force=1e-4),
+}
+
+
+@pytest.mark.parametrize("name", list(AA_CASES))
+@pytest.mark.parametrize("steps", [7, 8])
+def test_aa_f64_matches_oracle(name, steps):
+    sp = AA_CASES[name].replace(precision=abi.F64, streaming=abi.STREAM_AA)
+    print(f"[aa-f64] {name} steps={steps}", flush=True)
+    g, o = Case(sp), oracle.OracleCase(sp)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    g.main_steps(steps, residuals=True)
+    o.main_steps(steps, residuals=True)
+    assert rel(g.pops(), o.pops()) < 1e-12
+    rg, ug = g.fields()
+    ro, uo = o.fields()
+    assert rel(rg, ro) < 1e-12 and rel(ug, uo) < 1e-11
+    r1, u1, _ = g.residuals()
+    r2, u2, _ = o.residuals()
+    assert abs(r1 - r2) <= 1e-9 * max(r2, 1e-30)
+
+
+@pytest.mark.parametrize("name", list(AA_CASES))
+def test_aa_f32_and_f32_shifted(name):
+    sp = AA_CASES[name]
+    o = oracle.OracleCase(sp.replace(precision=abi.F64))
+    o.initialize_nodes()
+    o.main_steps(20)
+    ro, uo = o.fields()
+    for prec in (abi.F32, abi.F32_SHIFTED):
+        print(f"[aa-f32] {name} prec={prec}", flush=True)
+        g = Case(sp.replace(precision=prec, streaming=abi.STREAM_AA))
+        g.initialize_nodes()
+        g.main_steps(20)
+        assert rel(g.pops(), o.pops()) < 1e-5
+        rg, ug = g.fields()
+        assert rel(rg, ro) < 1e-5
+        assert l2rel(ug, uo) < 1e-3 if prec == abi.F32 else l2rel(ug, uo) < 1e-4
+
+
+def test_shifted_half_precision_velocity_error():
+    sp = configs.porous_channel(n=(32, 24, 24), force=1e-5)
+    o = oracle.OracleCase(sp.replace(precision=abi.F64))
+    o.initialize_nodes()
+    o.main_steps(60)
+    ro, uo = o.fields()
+    mo = o.reduce(abi.REDUCE_TOTAL_MASS)
+    errs = {}
+    for prec in (abi.F16_SHIFTED, abi.BF16_SHIFTED):
+        g = Case(sp.replace(precision=prec))
+        g.initialize_nodes()
+        g.main_steps(60)
+        rg, ug = g.fields()
+        errs[prec] = l2rel(ug, uo)
+        drift = abs(g.reduce(abi.REDUCE_TOTAL_MASS) - mo) / mo
+        print(f"[shifted] prec={prec} L2 u err {errs[prec]:.3e} mass drift {drift:.3e}", flush=True)
+        assert drift < 1e-4
+        assert g.reduce(abi.REDUCE_NONFINITE) == 0
+    assert errs[abi.F16_SHIFTED] < 1e-3
+    assert errs[abi.BF16_SHIFTED] < 3e-2
+
+
+def test_aa_set_pops_odd_and_even():
+    sp = configs.porous_channel(n=(12, 10, 8), force=1e-4).replace(precision=abi.F64,
+                                                                    streaming=abi.STREAM_AA)
+    g, o = Case(sp), oracle.OracleCase(sp)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    g.main_steps(3)
+    o.main_steps(3)
+    f = o.pops()
+    g.set_pops(f)
+    assert np.array_equal(g.pops()[~sp.node_type.astype(bool)], f[~sp.node_type.astype(bool)])
+    g.main_steps(5)
+    o.main_steps(5)
+    assert rel(g.pops(), o.pops()) < 1e-12
