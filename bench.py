@@ -159,8 +159,8 @@ def main():
                 ("f32_two_array", abi.F32, abi.STREAM_TWO_ARRAY, 4),
                 ("f64_aa", abi.F64, abi.STREAM_AA, 8)]
     if args.only:
-        keep = set(args.only.split(","))
-        variants = [v for v in variants if v[0] in keep]
+        by_name = {v[0]: v for v in variants}
+        variants = [by_name[k] for k in args.only.split(",")]
     results = {}
     copy_gbs = copy_bandwidth(hip)
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)

@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -187,6 +188,7 @@ struct lbf_case {
   bool aa = false;       // in-place AA scheme (one population array)
   uint64_t host_step = 0;  // AA: steps enqueued (device count, resynchronised on reads)
   bool aa_fresh = false;   // AA: array holds post-stream f (after init / set_pops)
+  int xcd_slab = 0;        // AA: XCD-contiguous workgroup order
   bool may_have_stopped = false;  // steps were enqueued with the stop rule on
   double scale = 1.0;    // shifted storage: stored = (f - w) * scale
   double w[27] = {};
@@ -473,6 +475,7 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   // one node per thread: the streaming kernels have no loop-carried state and
   // every workgroup writes its own residual partials
   c->grid = (int)std::max(1LL, blocks);
+  c->xcd_slab = 0;  // measured slower than round-robin on the porous 256^3 case
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
   auto alloc = [&](void** p, size_t bytes) -> int {
@@ -602,6 +605,7 @@ KArgs momentum_args(lbf_case* c) {
   a.plane_bytes = (uint32_t)(c->ld * c->sb);
   a.fresh = c->aa_fresh ? 1 : 0;
   a.aa_odd = (int)(c->host_step & 1);
+  a.xcd_slab = c->aa ? c->xcd_slab : 0;
   a.inv_scale = 1.0 / c->scale;
   a.faces_noslip_only = 1;
   for (int f = 0; f < 2 * c->d; f++)

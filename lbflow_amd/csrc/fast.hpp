@@ -7,13 +7,23 @@
 // and written by the same thread within a step, so the update is in place:
 // half the memory of a two-array scheme at the same 2*Q*s bytes per update.
 //
-// Links whose neighbour is solid, or that cross a face with a non-periodic
-// condition, are redirected to the node's own slots (the value that would be
-// pulled back is the node's own f*_i, i.e. bounce-back), so the post-stream
-// population arriving through such a link is f_ibar = f*_i; face conditions
-// are then applied in face order from those values (src/momentum/bc.rs:
-// 29-129). This requires opposite faces to be both periodic or both not,
-// which the host checks before choosing this scheme.
+// Bounce-back: the population arriving through a link whose neighbour is
+// solid, or that crosses a face with a non-periodic condition, is
+// f_ibar = f*_i (the node's own post-collision value sent back).
+//  - Links across non-periodic faces: the odd step writes f*_i to the node's
+//    own slot F[ibar][x] instead of F[i][x + c_i] (which is another node's).
+//  - Links to a solid node y = x + c_i: the odd step writes f*_i to
+//    F[i][y] as for a fluid neighbour (y never runs, so only x writes that
+//    slot), and the following even step reads f_ibar(x) from there. Every
+//    store of both steps then goes to a cache line that one wavefront writes
+//    whole. Stores redirected into lines that another wavefront also writes
+//    cost up to 1.6x the write traffic on a 30 %-solid lattice (measured);
+//    the extra loads of this layout cost less.
+//  - The even step writes only own slots, so the odd step reads a solid
+//    link's value from the node's own slot F[i][x].
+// Face conditions are then applied in face order from those values
+// (src/momentum/bc.rs:29-129). This requires opposite faces to be both
+// periodic or both not, which the host checks before choosing this scheme.
 //
 // Addressing: one buffer resource over the whole population array, a 32-bit
 // per-thread byte offset (shared by all planes of a neighbour) and a
@@ -108,6 +118,14 @@ __device__ __forceinline__ void bstore(S v, __amdgpu_buffer_rsrc_t r, uint32_t v
     __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, v), r, voff, soff, 0);
 }
 
+// A per-lane byte offset the compiler cannot see through: without it,
+// load(c ? a : b) is turned into branches around two loads with a wait
+// inside each, which serialises the gather on mixed wavefronts.
+__device__ __forceinline__ uint32_t opaque(uint32_t x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // directions crossing face F (c . n_out = +1)
 template <class L>
 constexpr uint32_t face_dirs(int F) {
@@ -154,39 +172,50 @@ __device__ __forceinline__ void node_pos(const KArgs& a, uint32_t n, bool need_o
   }
 }
 
+// directions crossing the non-periodic faces the node lies on
 template <class L>
-__device__ __forceinline__ uint32_t redirect_mask(uint32_t cls, uint32_t on) {
-  uint32_t red = cls & kLinkMask;
+__device__ __forceinline__ uint32_t face_mask(uint32_t on) {
+  uint32_t m = 0;
   if (on) {
     sfor<2 * L::D>([&](auto FI) {
       constexpr int Fc = decltype(FI)::value;
-      if (on >> Fc & 1u) red |= face_dirs<L>(Fc);
+      if (on >> Fc & 1u) m |= face_dirs<L>(Fc);
     });
   }
-  return red;
+  return m;
 }
 
 // Post-stream populations of node n (compute type, delta form when shifted).
+// links: bit i = neighbour x + c_i (periodic wrap) is solid; fm: face_mask.
 template <class L, class St, bool ODD>
 __device__ __forceinline__ void aa_gather(__amdgpu_buffer_rsrc_t rs, uint32_t plane_bytes,
-                                          uint32_t n, bool fresh, uint32_t red, const NodePos& p,
-                                          typename St::T* f) {
+                                          uint32_t n, bool fresh, uint32_t links, uint32_t fm,
+                                          const NodePos& p, typename St::T* f) {
   using S = typename St::S;
   using T = typename St::T;
   using k = K<L>;
   const uint32_t own = n * (uint32_t)sizeof(S);
+  // odd: links whose value is in the node's own slot; even: solid links
+  // whose value the previous odd step left in the solid neighbour's slot
+  const uint32_t sel = ODD ? (links | fm) : (links & ~fm);
   sfor<L::Q>([&](auto J) {
     constexpr int j = decltype(J)::value;
     constexpr int jb = k::qbar(j);
     S v;
-    if (!ODD || fresh) {
+    if (fresh) {
       v = bload<S>(rs, own, j * plane_bytes);
+    } else if constexpr (!ODD) {
+      // f_j(x) = f*_jb(x), stored at F[jb][x + c_jb] when that node is solid
+      const int32_t d = p.ox[1 + k::cx(jb)] + p.oy[1 + k::cy(jb)] + p.oz[1 + k::cz(jb)];
+      const uint32_t nb = (uint32_t)((int32_t)n + d) * (uint32_t)sizeof(S) + jb * plane_bytes;
+      const uint32_t mine = own + j * plane_bytes;
+      v = bload<S>(rs, opaque((sel >> jb & 1u) ? nb : mine), 0u);
     } else {
       // one load per direction: the redirect only selects the lane's address
       const int32_t d = p.ox[1 - k::cx(j)] + p.oy[1 - k::cy(j)] + p.oz[1 - k::cz(j)];
       const uint32_t nb = (uint32_t)((int32_t)n + d) * (uint32_t)sizeof(S) + jb * plane_bytes;
       const uint32_t mine = own + j * plane_bytes;
-      v = bload<S>(rs, (red >> jb & 1u) ? mine : nb, 0u);
+      v = bload<S>(rs, opaque((sel >> jb & 1u) ? mine : nb), 0u);
     }
     f[j] = T(v);  // shifted storage: scaled by the caller
   });
@@ -251,18 +280,21 @@ __global__ void __launch_bounds__(kBlock)
   const T sc = T(a.scale), isc = T(a.inv_scale);
   const uint32_t N = (uint32_t)a.N;
   const uint32_t ld = (uint32_t)a.ld;
-  const bool any_face = a.face_active != 0;
   double acc[5] = {0, 0, 0, 0, 0};
-  // one node per thread (the host launches ceil(N / kBlock) workgroups)
-  const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
+  // one node per thread (the host launches ceil(N / kBlock) workgroups).
+  // xcd_slab: workgroups are dealt round-robin to the 8 XCDs, so workgroup b
+  // takes node block (b % 8) * (grid / 8) + b / 8 and each XCD sweeps one
+  // contiguous slab; y neighbours then share that XCD's L2 (host: grid % 8 == 0).
+  uint32_t blk = blockIdx.x;
+  if (a.xcd_slab) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
+  const uint32_t n = blk * kBlock + threadIdx.x;
   const uint32_t cls = n < N ? a.cls[n] : kSolidBit;
   if (!(cls & kSolidBit)) {
     NodePos p;
-    p.on = 0;
-    if (ODD || any_face) node_pos<L>(a, n, ODD, p);
-    const uint32_t red = ODD ? redirect_mask<L>(cls, p.on) : 0u;
+    node_pos<L>(a, n, true, p);
+    const uint32_t fm = face_mask<L>(p.on);
     T f[Q];
-    aa_gather<L, St, ODD>(rs, pb, n, fresh, red, p, f);
+    aa_gather<L, St, ODD>(rs, pb, n, fresh, cls & kLinkMask, fm, p, f);
     if constexpr (St::shifted)
       sfor<Q>([&](auto J) { f[decltype(J)::value] *= isc; });
     if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T>(a, p.on, f);
@@ -347,7 +379,7 @@ __global__ void __launch_bounds__(kBlock)
         const int32_t d = p.ox[1 + k::cx(j)] + p.oy[1 + k::cy(j)] + p.oz[1 + k::cz(j)];
         const uint32_t nb = (uint32_t)((int32_t)n + d) * (uint32_t)sizeof(S) + j * pb;
         const uint32_t mine = own + jb * pb;
-        bstore<S>(sv, rs, (red >> j & 1u) ? mine : nb, 0u);
+        bstore<S>(sv, rs, opaque((fm >> j & 1u) ? mine : nb), 0u);
       }
     });
   }
@@ -379,9 +411,8 @@ __global__ void __launch_bounds__(kBlock) momentum_readback_aa(KArgs a) {
     if (cls & kSolidBit) continue;
     NodePos p;
     node_pos<L>(a, n, true, p);
-    const uint32_t red = redirect_mask<L>(cls, p.on);
     T f[Q];
-    aa_gather<L, St, ODD>(rs, a.plane_bytes, n, fresh, red, p, f);
+    aa_gather<L, St, ODD>(rs, a.plane_bytes, n, fresh, cls & kLinkMask, face_mask<L>(p.on), p, f);
     if constexpr (St::shifted)
       sfor<Q>([&](auto J) { f[decltype(J)::value] *= T(a.inv_scale); });
     if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T>(a, p.on, f);
