@@ -25,6 +25,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per step launch measured with rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE),
+# calibrated on the plain copy kernel; written by lbflow_amd.profiling.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_d3q19_trt_porous_256.txt")
 METRIC = "MLUPS & % HBM peak, D3Q19 256³; served cases/s + p99 latency vs plain f64"
 
 
@@ -110,7 +113,7 @@ def time_case(spec, steps, warmup, hip):
     return ms, wall, nonfinite, mass
 
 
-def cpu_baseline(n=128, steps=150, threads=None):
+def cpu_baseline(n=128, steps=450, threads=None):
     """Times the oracle on a bounded sample (n^3 porous channel, same physics)."""
     from lbflow_amd import configs
     from oracle import oracle
@@ -182,6 +185,8 @@ def main():
     head_name = "f32" if "f32" in results else next(iter(results))
     cpu = None if args.no_cpu else cpu_baseline()
     bytes_per_launch = N * head["bytes_per_site"]
+    from lbflow_amd.profiling import read_traffic
+    traffic = read_traffic(TRAFFIC_FILE)
     line = {
         "metric": METRIC,
         "value": head["mlups"],
@@ -200,8 +205,11 @@ def main():
                                f"{head_name} populations, in-place AA-pattern fused kernel",
                    "lattice": "D3Q19", "nodes": N, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": head["achieved_gbs"], "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": head["hbm_frac"], "traffic": None,
-                     "algorithmic_bytes_per_launch": bytes_per_launch},
+                     "unit": "GB/s", "frac": head["hbm_frac"],
+                     "traffic": traffic.get(head_name) if n == 256 else None,
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "attainable_copy_gbs": round(copy_gbs, 1),
+                     "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4)},
         "cpu_baseline": cpu,
         "extra": dict(results, copy_gbs=round(copy_gbs, 1)),
     }

