@@ -148,7 +148,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              f"(every case fits one MI355X; sharding is described in DESIGN.md)", file=sys.stderr)
+              f"(every case fits one MI355X's HBM; multi-GPU sharding is not built)", file=sys.stderr)
         sys.exit(2)
 
     from lbflow_amd import abi, configs

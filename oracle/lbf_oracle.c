@@ -19,13 +19,13 @@
  * over the fluid nodes with an implicit barrier between them, exactly like
  * the rayon `par_iter` phases of src/momentum/lattice.rs:330-367.
  *
- * Deliberate fixes of reference nondeterminism (recorded in DESIGN.md):
+ * Deliberate fixes of reference nondeterminism:
  *  - face conditions run in FACES_2D/FACES_3D order (src/lib.rs:27-29)
  *    instead of HashMap order (src/momentum/bc.rs:30-32);
  *  - residual sums run sequentially in node order (rayon's order varies).
  *
  * Parity status of every function is listed in the test suite header
- * (tests/test_oracle_kat.py); functions without a known-answer test of the
+ * (tests/test_oracle.py); functions without a known-answer test of the
  * reference are pinned by closed forms, invariants and brute force there.
  */
 #include "../include/lbflow_hip.h"
