@@ -137,6 +137,127 @@ def cpu_baseline(n=128, steps=450, threads=None):
                       f"({dt:.1f} s)"}
 
 
+def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100,
+                latency_sample=200, seed=5):
+    """Serving mix on the batched server versus sequential plain f64 runs.
+
+    Mix (configs.serving_mix): D2Q9 / D3Q19 50/50, sides log-uniform in
+    [32, 256] (D2Q9) and [32, side_3d_max] (D3Q19), tau ~ U[0.55, 1.2],
+    steps ~ U{1000, 20000} x step_scale, porosity ~ U[0.6, 0.9], FP32 or
+    FP16-shifted 50/50, periodic faces, BGK.
+      - throughput: all `count` requests submitted at once, served cases/s;
+        the plain baseline runs a uniform random sample of `baseline_sample`
+        cases one after another (create, f64 two-array steps, fields) and
+        scales its time to the whole mix;
+      - latency: `latency_sample` cases arrive as a Poisson process at 1.2x
+        the plain path's capacity; served time-to-solution is measured live,
+        the plain FIFO queue's from the measured service times of the same
+        cases (finish_i = max(arrival_i, finish_i-1) + service_i).
+    """
+    import numpy as np
+    from lbflow_amd import abi, configs
+    from lbflow_amd.engine import Case
+    from lbflow_amd.serve import Server
+
+    params = configs.serving_mix(count, seed=seed, side_2d=(32, 256), side_3d=(32, side_3d_max),
+                                 steps=(int(1000 * step_scale), int(20000 * step_scale)))
+
+    def spec_of(p, prec=None):
+        return configs.serving_case(p["velocity_set"], p["side"], p["tau"], p["porosity"],
+                                    p["seed"], p["precision"] if prec is None else prec)
+
+    specs = [(spec_of(p), p["steps"]) for p in params]
+    nodes_steps = sum(sp.num_nodes * n for sp, n in specs)
+    rng = np.random.default_rng(seed + 1)
+
+    def plain_service(idx):
+        times = []
+        for i in idx:
+            sp = spec_of(params[i], abi.F64).replace(streaming=abi.STREAM_TWO_ARRAY)
+            t0 = time.perf_counter()
+            c = Case(sp)
+            c.initialize_nodes()
+            c.main_steps(params[i]["steps"])
+            c.fields()
+            c.close()
+            times.append(time.perf_counter() - t0)
+        return np.array(times)
+
+    # plain baseline on a random sample, scaled to the whole mix
+    base_idx = rng.choice(count, size=min(baseline_sample, count), replace=False)
+    print(f"[serve] plain f64 baseline on {len(base_idx)} of {count} cases", file=sys.stderr,
+          flush=True)
+    base_t = plain_service(base_idx)
+    plain_total = base_t.sum() * count / len(base_idx)
+    plain_cps = count / plain_total
+
+    # saturated throughput
+    print(f"[serve] served: {count} cases at once", file=sys.stderr, flush=True)
+    t0 = time.perf_counter()
+    with Server() as srv:
+        tickets = [srv.submit(sp, n) for sp, n in specs]
+        srv.drain(3600)
+        st = srv.stats()
+        for t in tickets:
+            srv.result(t)
+    wall = time.perf_counter() - t0
+    served_cps = count / wall
+
+    # latency under Poisson arrivals at 1.2x the plain capacity
+    lat_idx = rng.choice(count, size=min(latency_sample, count), replace=False)
+    svc = plain_service(lat_idx)
+    rate = 1.2 * len(lat_idx) / svc.sum()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(lat_idx)))
+    fin = np.zeros(len(lat_idx))
+    prev = 0.0
+    for k in range(len(lat_idx)):
+        prev = max(arrivals[k], prev) + svc[k]
+        fin[k] = prev
+    plain_lat = fin - arrivals
+    print(f"[serve] Poisson arrivals: {len(lat_idx)} cases at {rate:.1f}/s", file=sys.stderr,
+          flush=True)
+    with Server() as srv:
+        start = time.perf_counter()
+        tickets = []
+        for k, i in enumerate(lat_idx):
+            delay = arrivals[k] - (time.perf_counter() - start)
+            if delay > 0:
+                time.sleep(delay)
+            tickets.append(srv.submit(specs[i][0], specs[i][1]))
+        srv.drain(3600)
+        lat = []
+        for t in tickets:
+            inf = srv.info(t)
+            lat.append(inf.t_done - inf.t_submit)
+            srv.result(t)
+        st_lat = srv.stats()
+    lat = np.array(lat)
+    return {
+        "metric": "served cases/s and time to solution, serving mix",
+        "cases": count,
+        "mix": f"D2Q9/D3Q19 50/50, sides log-uniform [32,256] 2-D and [32,{side_3d_max}] 3-D, "
+               f"tau U[0.55,1.2], steps U{{{int(1000 * step_scale)},{int(20000 * step_scale)}}} "
+               f"(the mix's 1,000-20,000 x {step_scale}), porosity U[0.6,0.9], FP32/FP16-shifted "
+               f"50/50, periodic, BGK, body force 1e-5",
+        "node_updates": float(nodes_steps),
+        "served_cases_per_s": round(served_cps, 2),
+        "served_mlups": round(nodes_steps / wall / 1e6, 1),
+        "plain_f64_cases_per_s": round(plain_cps, 2),
+        "plain_f64_sample": int(len(base_idx)),
+        "throughput_ratio": round(served_cps / plain_cps, 3),
+        "gpu_busy_frac": round(st["gpu_seconds"] / wall, 3),
+        "hbm_peak_bytes": int(st["hbm_peak"]),
+        "tuned_grid": int(st["tuned_grid"]),
+        "poisson_rate_per_s": round(rate, 2),
+        "latency_cases": int(len(lat_idx)),
+        "served_p50_s": round(float(np.percentile(lat, 50)), 4),
+        "served_p99_s": round(float(np.percentile(lat, 99)), 4),
+        "plain_p50_s": round(float(np.percentile(plain_lat, 50)), 4),
+        "plain_p99_s": round(float(np.percentile(plain_lat, 99)), 4),
+        "graphs_built": int(st["graphs_built"] + st_lat["graphs_built"]),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +266,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="comma list of variants (f32,f64,f16s,...)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix leg")
+    ap.add_argument("--serve-cases", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -183,6 +306,7 @@ def main():
               file=sys.stderr, flush=True)
     head = results.get("f32") or next(iter(results.values()))
     head_name = "f32" if "f32" in results else next(iter(results))
+    serving = None if args.no_serve else serving_leg(count=args.serve_cases)
     cpu = None if args.no_cpu else cpu_baseline()
     bytes_per_launch = N * head["bytes_per_site"]
     from lbflow_amd.profiling import read_traffic
@@ -211,6 +335,7 @@ def main():
                      "attainable_copy_gbs": round(copy_gbs, 1),
                      "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4)},
         "cpu_baseline": cpu,
+        "serving": serving,
         "extra": dict(results, copy_gbs=round(copy_gbs, 1)),
     }
     print(json.dumps(line))

@@ -42,6 +42,7 @@ extern "C" {
 #define LBF_ENOMEM (-4)      /* device or host allocation failed / HBM budget */
 #define LBF_ESTATE (-5)      /* call not valid in the handle's current state  */
 #define LBF_ENONFINITE (-6)  /* a NaN/Inf was detected in the fields          */
+#define LBF_ETIMEOUT (-7)    /* a wait ended before the request completed     */
 
 /* ---- velocity sets: discriminants of `VelocitySet` (src/velocity_set.rs:18-25) */
 #define LBF_D2Q5 0 /* not constructible in the reference (todo!, velocity_set.rs:50) */

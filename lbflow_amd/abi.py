@@ -11,7 +11,7 @@ MAX_Q = 27
 MAX_SCALARS = 4
 
 # error codes
-OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE, ENONFINITE = 0, -1, -2, -3, -4, -5, -6
+OK, EINVAL, EUNSUPPORTED, EDEVICE, ENOMEM, ESTATE, ENONFINITE, ETIMEOUT = 0, -1, -2, -3, -4, -5, -6, -7
 
 # velocity sets: `VelocitySet` discriminants (src/velocity_set.rs:18-25)
 D2Q5, D2Q9, D3Q15, D3Q19, D3Q27 = 0, 1, 2, 3, 4
@@ -115,6 +115,51 @@ ENTRY_POINTS = [
     ("lbf_get_link_tables", C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), _u8p]),
     ("lbf_device_bytes", C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_uint64)]),
     ("lbf_last_error", C.c_char_p, []),
+]
+
+
+# ---- serving tier (include/lbflow_serve.h) ---------------------------------
+SERVE_KEEP_POPS = 1
+REQ_QUEUED, REQ_PREPARING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(5)
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("abi_version", C.c_uint32), ("max_slots", C.c_uint32),
+                ("hbm_budget_bytes", C.c_uint64), ("max_steps_per_round", C.c_uint32),
+                ("autotune", C.c_int32), ("grid", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("error", C.c_int32), ("nodes", C.c_uint64),
+                ("steps", C.c_uint64), ("device_bytes", C.c_uint64), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("queued", C.c_uint64), ("resident", C.c_uint64), ("rounds", C.c_uint64),
+                ("graphs_built", C.c_uint64), ("hbm_in_use", C.c_uint64),
+                ("hbm_peak", C.c_uint64), ("hbm_budget", C.c_uint64),
+                ("node_updates", C.c_double), ("first_submit", C.c_double),
+                ("last_done", C.c_double), ("gpu_seconds", C.c_double),
+                ("p50_latency", C.c_double), ("p99_latency", C.c_double),
+                ("mean_latency", C.c_double), ("cases_per_second", C.c_double),
+                ("mlups", C.c_double), ("prep_seconds", C.c_double),
+                ("retire_seconds", C.c_double), ("tuned_grid", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+SERVER_ENTRY_POINTS = [
+    ("lbf_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
+    ("lbf_server_destroy", None, [_vp]),
+    ("lbf_server_submit", C.c_int, [_vp, C.POINTER(CaseDesc), C.c_uint64, C.c_uint32,
+                                    C.POINTER(C.c_uint64)]),
+    ("lbf_server_info", C.c_int, [_vp, C.c_uint64, C.POINTER(RequestInfo)]),
+    ("lbf_server_wait", C.c_int, [_vp, C.c_uint64, C.c_double]),
+    ("lbf_server_result", C.c_int, [_vp, C.c_uint64, _dp, _dp, _dp, C.c_int32]),
+    ("lbf_server_drain", C.c_int, [_vp, C.c_double]),
+    ("lbf_server_get_stats", C.c_int, [_vp, C.POINTER(ServerStats)]),
+    ("lbf_server_now", C.c_double, [_vp]),
 ]
 
 

@@ -76,3 +76,39 @@ def sphere_channel(nx=384, ny=128, nz=128, radius=16, u_in=0.05, tau=0.6, precis
         velocity_set=velocity_set, n=(nx, ny, nz), collision=op or Collision.bgk(tau),
         faces=[FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 4, node_type=mask,
         initial_density=1.0, initial_velocity=(u_in, 0.0, 0.0), precision=precision)
+
+
+def serving_case(velocity_set=abi.D2Q9, side=64, tau=0.8, porosity=0.8, seed=0,
+                 precision=abi.F32, force=1e-5) -> Spec:
+    """One case of the serving mix: BGK(tau), all faces periodic, a
+    Bernoulli(1 - porosity) solid mask, a small body force along x so the
+    flow is not at rest (force is this build's choice; the mix leaves it open)."""
+    d = abi.DIM_OF[velocity_set]
+    shape = (side,) * d
+    mask = bernoulli_mask(shape, 1.0 - porosity, seed)
+    return Spec(
+        velocity_set=velocity_set, n=shape, collision=Collision.bgk(tau),
+        faces=[FaceBC.periodic()] * (2 * d), node_type=mask,
+        force=(force,) + (0.0,) * (d - 1), initial_density=1.0,
+        initial_velocity=(0.0,) * d, precision=precision)
+
+
+def serving_mix(count=2000, seed=5, side_2d=(32, 256), side_3d=(32, 256),
+                steps=(1000, 20000)):
+    """The serving mix: lattice D2Q9 / D3Q19 50/50, sides log-uniform in the
+    given ranges, tau ~ U[0.55, 1.2], steps ~ U{steps}, porosity ~ U[0.6, 0.9],
+    storage FP32 or FP16-shifted 50/50. Returns [(spec, steps)] lazily built
+    from per-case parameters (cases are generated when iterated)."""
+    rng = np.random.default_rng(seed)
+    params = []
+    for i in range(count):
+        vs = abi.D2Q9 if rng.random() < 0.5 else abi.D3Q19
+        lo, hi = side_2d if vs == abi.D2Q9 else side_3d
+        side = int(round(np.exp(rng.uniform(np.log(lo), np.log(hi)))))
+        tau = float(rng.uniform(0.55, 1.2))
+        nsteps = int(rng.integers(steps[0], steps[1] + 1))
+        porosity = float(rng.uniform(0.6, 0.9))
+        prec = abi.F32 if rng.random() < 0.5 else abi.F16_SHIFTED
+        params.append(dict(velocity_set=vs, side=side, tau=tau, porosity=porosity,
+                           seed=seed * 100003 + i, precision=prec, steps=nsteps))
+    return params

@@ -14,11 +14,13 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
 #include "../../include/lbflow_hip.h"
 #include "dispatch.hpp"
+#include "internal.hpp"
 #include "engine.hpp"
 
 using namespace lbf;
@@ -196,18 +198,50 @@ struct lbf_case {
   bool needs_moments = false;  // some condition reads stored moments every step
   uint32_t face_active = 0;
   // host copies of the node classes
-  std::vector<uint8_t> h_type;
-  std::vector<uint32_t> h_mask;
-  std::vector<uint8_t> h_faces;
+  std::vector<uint8_t> h_type;  // node types (host copy)
+  long long n_fluid = 0;
   KArgs base{};
 };
 
 namespace {
 
+// Case buffers come from the device's stream-ordered memory pool when the
+// device has one: creating and releasing a case then costs no device-wide
+// synchronisation (hipFree waits for the whole device), which matters when a
+// server turns cases over while others run. The pool keeps released memory
+// for reuse instead of returning it to the driver.
+bool use_pool(int device) {
+  static std::once_flag once[64];
+  static bool ok[64] = {};
+  if (device < 0 || device >= 64) return false;
+  std::call_once(once[device], [device] {
+    int supported = 0;
+    if (hipDeviceGetAttribute(&supported, hipDeviceAttributeMemoryPoolsSupported, device) !=
+            hipSuccess ||
+        !supported)
+      return;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess) return;
+    uint64_t keep = UINT64_MAX;
+    ok[device] = hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep) == hipSuccess;
+  });
+  return ok[device];
+}
+
+hipError_t dev_alloc(lbf_case* c, void** p, size_t bytes) {
+  if (c->stream && use_pool(c->device)) return hipMallocAsync(p, bytes, c->stream);
+  return hipMalloc(p, bytes);
+}
+
 void free_case(lbf_case* c) {
   if (!c) return;
-  auto F = [](void* p) {
-    if (p) (void)hipFree(p);
+  const bool pooled = c->stream && use_pool(c->device);
+  auto F = [&](void* p) {
+    if (!p) return;
+    if (pooled)
+      (void)hipFreeAsync(p, c->stream);
+    else
+      (void)hipFree(p);
   };
   F(c->pop[0]), F(c->pop[1]), F(c->cls), F(c->rho[0]), F(c->rho[1]), F(c->u[0]), F(c->u[1]),
       F(c->force), F(c->ctrl), F(c->partials), F(c->readback), F(c->red_out);
@@ -421,46 +455,20 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     if (d->faces[f].kind == LBF_BC_ANTI_BOUNCE_BACK) c->needs_moments = true;
   if (c->ns) c->needs_moments = true;
 
-  // ---- node classes (lattice.rs:127-191): periodic neighbours, solid links
+  // ---- node types (the classes themselves are built on the device)
   const long long N = c->N;
   const int q = c->q;
   c->h_type.assign(N, 0);
-  if (d->node_type)
-    for (long long i = 0; i < N; i++) {
+  if (d->node_type) {
+    for (long long i = 0; i < N; i++)
       if (d->node_type[i] > 1) {
         free_case(c);
         return fail(LBF_EINVAL, "Invalid node type at node %lld", i);
       }
-      c->h_type[i] = d->node_type[i];
-    }
-  c->h_mask.assign(N, 0);
-  c->h_faces.assign(N, 0);
-  std::vector<uint32_t> cls(c->ld, kSolidBit);
-  const long long n3[3] = {c->nx, c->ny, c->nz};
-  for (long long i = 0; i < N; i++) {
-    const long long x = i % c->nx, y = (i / c->nx) % c->ny, z = i / ((long long)c->nx * c->ny);
-    const long long idx[3] = {x, y, z};
-    uint32_t m = 0;
-    if (!c->h_type[i]) {
-      for (int k = 0; k < q; k++) {
-        long long nb[3] = {x, y, z};
-        for (int a = 0; a < c->d; a++) nb[a] = ((idx[a] + v.c[k][a]) % n3[a] + n3[a]) % n3[a];
-        const long long j = nb[0] + n3[0] * (nb[1] + n3[1] * nb[2]);
-        if (c->h_type[j]) m |= 1u << k;
-      }
-      uint8_t fb = 0;
-      for (int f = 0; f < nf; f++) {
-        const int a = f / 2;
-        const long long lim = (f % 2) ? n3[a] - 1 : 0;
-        if (idx[a] == lim) fb |= (uint8_t)(1u << f);
-      }
-      c->h_faces[i] = fb;
-      c->h_mask[i] = m;
-      cls[i] = m;
-    } else {
-      cls[i] = kSolidBit;
-    }
+    memcpy(c->h_type.data(), d->node_type, N);
   }
+  c->n_fluid = 0;
+  for (long long i = 0; i < N; i++) c->n_fluid += c->h_type[i] == 0;
 
   // ---- device memory
   auto cleanup = [&](int code) {
@@ -468,9 +476,6 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     return code;
   };
   if (hipGetDevice(&c->device) != hipSuccess) return cleanup(fail(LBF_EDEVICE, "no HIP device"));
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, c->device) != hipSuccess)
-    return cleanup(fail(LBF_EDEVICE, "hipGetDeviceProperties failed"));
   const long long blocks = (N + kBlock - 1) / kBlock;
   // one node per thread: the streaming kernels have no loop-carried state and
   // every workgroup writes its own residual partials
@@ -478,11 +483,21 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   c->xcd_slab = 0;  // measured slower than round-robin on the porous 256^3 case
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
+  // buffers are cleared on the case stream; the stream is synchronised
+  // before any synchronous copy writes into them
   auto alloc = [&](void** p, size_t bytes) -> int {
-    if (hipMalloc(p, bytes) != hipSuccess)
-      return fail(LBF_ENOMEM, "hipMalloc(%zu) failed", bytes);
-    if (hipMemset(*p, 0, bytes) != hipSuccess) return fail(LBF_EDEVICE, "hipMemset failed");
+    if (dev_alloc(c, p, bytes) != hipSuccess)
+      return fail(LBF_ENOMEM, "device allocation of %zu bytes failed", bytes);
+    if (hipMemsetAsync(*p, 0, bytes, c->stream) != hipSuccess)
+      return fail(LBF_EDEVICE, "hipMemsetAsync failed");
     return 0;
+  };
+  // for buffers that synchronous copies fill right away
+  auto alloc_sync = [&](void** p, size_t bytes) -> int {
+    int r = alloc(p, bytes);
+    if (!r && hipStreamSynchronize(c->stream) != hipSuccess)
+      r = fail(LBF_EDEVICE, "hipStreamSynchronize failed");
+    return r;
   };
   const size_t pop_bytes = (size_t)q * c->ld * c->sb;
   for (int b = 0; b < 2; b++) {
@@ -491,20 +506,44 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     if ((rc = alloc(&c->u[b], (size_t)c->d * c->ld * c->tb))) return cleanup(rc);
   }
   if ((rc = alloc((void**)&c->cls, c->ld * 4))) return cleanup(rc);
-  if (hipMemcpy(c->cls, cls.data(), c->ld * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return cleanup(fail(LBF_EDEVICE, "upload of node classes failed"));
-  if ((rc = alloc((void**)&c->ctrl, sizeof(Ctrl)))) return cleanup(rc);
   {
-    Ctrl h{};
-    h.max_iter = ~0ull;
-    if (hipMemcpy(c->ctrl, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess)
-      return cleanup(fail(LBF_EDEVICE, "upload of control block failed"));
+    // node classes on the device; node_type is staged in rho[1], which the
+    // initial moments overwrite afterwards on the same stream
+    ClassArgs ca{};
+    ca.nx = c->nx, ca.ny = c->ny, ca.nz = c->nz, ca.q = q;
+    ca.N = N, ca.ld = c->ld;
+    for (int k = 0; k < q; k++)
+      for (int a = 0; a < 3; a++) ca.c[k][a] = a < c->d ? v.c[k][a] : 0;
+    const uint8_t* type_dev = nullptr;
+    if (d->node_type) {
+      if (hipMemcpyAsync(c->rho[1], d->node_type, N, hipMemcpyHostToDevice, c->stream) !=
+          hipSuccess)
+        return cleanup(fail(LBF_EDEVICE, "upload of node types failed"));
+      type_dev = (const uint8_t*)c->rho[1];
+    }
+    if (launch_build_classes(ca, type_dev, c->cls, c->stream) != hipSuccess)
+      return cleanup(fail(LBF_EDEVICE, "building node classes failed"));
   }
+  if ((rc = alloc((void**)&c->ctrl, sizeof(Ctrl)))) return cleanup(rc);
+  if (launch_ctrl_set(c->ctrl, kCtrlMaxIter, -1, c->stream) != hipSuccess)  // max_iter = ~0
+    return cleanup(fail(LBF_EDEVICE, "control block initialisation failed"));
   if ((rc = alloc((void**)&c->partials, (size_t)c->grid * kMaxCols * 8))) return cleanup(rc);
   if ((rc = alloc((void**)&c->red_out, 64))) return cleanup(rc);
 
   // initial moments into both buffers (solid nodes keep them forever)
-  {
+  if (!d->initial_density && !d->initial_velocity) {
+    bool ok = true;
+    for (int b = 0; b < 2; b++) {
+      ok = ok && launch_fill(c->rho[b], c->tb == 8, N, c->ld, d->initial_density_uniform,
+                             c->stream) == hipSuccess;
+      for (int a = 0; a < c->d; a++)
+        ok = ok && launch_fill((char*)c->u[b] + (size_t)a * c->ld * c->tb, c->tb == 8, N, c->ld,
+                               d->initial_velocity_uniform[a], c->stream) == hipSuccess;
+    }
+    if (!ok) return cleanup(fail(LBF_EDEVICE, "initial moments failed"));
+  } else {
+    if (hipStreamSynchronize(c->stream) != hipSuccess)
+      return cleanup(fail(LBF_EDEVICE, "case set-up failed"));
     std::vector<double> rho(c->ld, 0.0), u((size_t)c->d * c->ld, 0.0);
     for (long long i = 0; i < N; i++) {
       rho[i] = d->initial_density ? d->initial_density[i] : d->initial_density_uniform;
@@ -517,11 +556,14 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
       if ((rc = upload_compute(c, c->u[b], u.data(), (size_t)c->d * c->ld))) return cleanup(rc);
     }
   }
+  // everything above ran on the case stream; later uploads are synchronous
+  if (hipStreamSynchronize(c->stream) != hipSuccess)
+    return cleanup(fail(LBF_EDEVICE, "case set-up failed"));
   if (d->force_kind == LBF_SOURCE_FIELD) {
     std::vector<double> F((size_t)c->d * c->ld, 0.0);
     for (long long i = 0; i < N; i++)
       for (int a = 0; a < c->d; a++) F[a * c->ld + i] = d->force_field[i * c->d + a];
-    if ((rc = alloc((void**)&c->force, F.size() * 8))) return cleanup(rc);
+    if ((rc = alloc_sync((void**)&c->force, F.size() * 8))) return cleanup(rc);
     if (hipMemcpy(c->force, F.data(), F.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(LBF_EDEVICE, "upload of force field failed"));
   }
@@ -532,8 +574,8 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     S.desc.source_field = nullptr;
     const lbf_scalar_desc& sd = d->scalars[s];
     for (int b = 0; b < 2; b++) {
-      if ((rc = alloc(&S.pop[b], (size_t)q * c->ld * c->tb))) return cleanup(rc);
-      if ((rc = alloc(&S.c[b], c->ld * c->tb))) return cleanup(rc);
+      if ((rc = alloc_sync(&S.pop[b], (size_t)q * c->ld * c->tb))) return cleanup(rc);
+      if ((rc = alloc_sync(&S.c[b], c->ld * c->tb))) return cleanup(rc);
     }
     std::vector<double> cv(c->ld, 0.0);
     for (long long i = 0; i < N; i++)
@@ -542,16 +584,15 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
       if ((rc = upload_compute(c, S.c[b], cv.data(), c->ld))) return cleanup(rc);
     if (sd.has_adsorption) {
       // q_ads init: initial value on momentum bounce-back nodes, else 0 (node.rs:77-85)
-      std::vector<double> qa(c->ld, 0.0);
-      for (long long i = 0; i < N; i++) qa[i] = c->h_mask[i] ? sd.initial_q_ads : 0.0;
-      if ((rc = alloc((void**)&S.q_ads, c->ld * 8))) return cleanup(rc);
-      if (hipMemcpy(S.q_ads, qa.data(), c->ld * 8, hipMemcpyHostToDevice) != hipSuccess)
-        return cleanup(fail(LBF_EDEVICE, "upload of q_ads failed"));
+      if ((rc = alloc_sync((void**)&S.q_ads, c->ld * 8))) return cleanup(rc);
+      if (launch_qads_init(c->cls, S.q_ads, c->ld, sd.initial_q_ads, c->stream) != hipSuccess ||
+          hipStreamSynchronize(c->stream) != hipSuccess)
+        return cleanup(fail(LBF_EDEVICE, "q_ads initialisation failed"));
     }
     if (sd.source_kind == LBF_SOURCE_FIELD) {
       std::vector<double> sf(c->ld, 0.0);
       for (long long i = 0; i < N; i++) sf[i] = sd.source_field[i];
-      if ((rc = alloc((void**)&S.src, c->ld * 8))) return cleanup(rc);
+      if ((rc = alloc_sync((void**)&S.src, c->ld * 8))) return cleanup(rc);
       if (hipMemcpy(S.src, sf.data(), c->ld * 8, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(LBF_EDEVICE, "upload of scalar source failed"));
     }
@@ -820,11 +861,7 @@ int lbf_reduce(lbf_case* c, int32_t which, double* out) {
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   HIPCHK(hipMemcpy(&r, c->red_out, 8, hipMemcpyDeviceToHost));
-  if (kind == LBF_REDUCE_MEAN_DENSITY) {
-    long long nf = 0;
-    for (long long i = 0; i < c->N; i++) nf += c->h_type[i] == 0;
-    r = r / (double)nf;
-  }
+  if (kind == LBF_REDUCE_MEAN_DENSITY) r = r / (double)c->n_fluid;
   *out = r;
   return 0;
 }
@@ -881,7 +918,7 @@ int lbf_get_pops(lbf_case* c, int32_t field, double* out) {
     int rc = read_ctrl(c, &h);
     if (rc) return rc;
   }
-  if (!c->readback) HIPCHK(hipMalloc((void**)&c->readback, bytes));
+  if (!c->readback) HIPCHK(dev_alloc(c, (void**)&c->readback, bytes));
   HIPCHK(hipMemsetAsync(c->readback, 0, bytes, c->stream));
   KArgs a = field == 0 ? momentum_args(c) : scalar_args(c, field - 1);
   a.mode = kModeStreamOnly;
@@ -937,8 +974,25 @@ int lbf_get_link_tables(lbf_case* c, uint8_t* node_type, uint32_t* bb_link_mask,
                         uint8_t* face_bits) {
   if (!c) return fail(LBF_EINVAL, "case is NULL");
   if (node_type) memcpy(node_type, c->h_type.data(), c->N);
-  if (bb_link_mask) memcpy(bb_link_mask, c->h_mask.data(), c->N * 4);
-  if (face_bits) memcpy(face_bits, c->h_faces.data(), c->N);
+  if (bb_link_mask || face_bits) {
+    std::vector<uint32_t> cls(c->ld);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(cls.data(), c->cls, c->ld * 4, hipMemcpyDeviceToHost));
+    const long long n3[3] = {c->nx, c->ny, c->nz};
+    for (long long i = 0; i < c->N; i++) {
+      const bool fluid = !(cls[i] & kSolidBit);
+      if (bb_link_mask) bb_link_mask[i] = fluid ? (cls[i] & kLinkMask) : 0u;
+      if (face_bits) {
+        const long long idx[3] = {i % c->nx, (i / c->nx) % c->ny, i / ((long long)c->nx * c->ny)};
+        uint8_t fb = 0;
+        for (int f = 0; fluid && f < 2 * c->d; f++) {
+          const long long lim = (f % 2) ? n3[f / 2] - 1 : 0;
+          if (idx[f / 2] == lim) fb |= (uint8_t)(1u << f);
+        }
+        face_bits[i] = fb;
+      }
+    }
+  }
   return 0;
 }
 
@@ -947,3 +1001,34 @@ int lbf_get_link_tables(lbf_case* c, uint8_t* node_type, uint32_t* bb_link_mask,
 // Benchmark hook (not part of the reference-equivalent ABI): the HIP stream
 // a case enqueues its launches on, so callers can bracket work with events.
 extern "C" void* lbfx_case_stream(lbf_case* c) { return c ? (void*)c->stream : nullptr; }
+
+// ---- internal interface for the serving tier (serve.hip) -----------------
+namespace lbf {
+
+int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
+
+int case_serve_info(lbf_case* c, CaseServeInfo* out) {
+  if (!c || !out) return fail(LBF_EINVAL, "NULL argument");
+  out->velocity_set = c->desc.velocity_set;
+  out->precision = c->prec;
+  out->op = c->desc.collision.op;
+  out->aa = c->aa;
+  out->n_scalars = c->ns;
+  out->N = c->N;
+  out->grid = c->grid;
+  out->stream = c->stream;
+  out->fresh = c->aa_fresh;
+  out->step = c->host_step;
+  out->args = momentum_args(c);
+  out->args.flags = kFlagStoreMoments;
+  out->args.mode = kModePull;
+  return 0;
+}
+
+void case_note_external_steps(lbf_case* c, uint64_t steps) {
+  if (!c || steps == 0) return;
+  c->host_step += steps;
+  c->aa_fresh = false;
+}
+
+}  // namespace lbf

@@ -8,6 +8,11 @@ namespace lbf {
 
 using LaunchFn = hipError_t (*)(const KArgs&, int grid, hipStream_t);
 
+struct ServeSlot;  // serve.hpp
+// batched step over a tile list (serving tier): slots, tiles, &ntiles, grid
+using ServeLaunchFn = hipError_t (*)(const ServeSlot*, const uint32_t*, const uint32_t*, int,
+                                     hipStream_t);
+
 struct Launchers {
   LaunchFn mom_step = nullptr;  // fused pull-stream / faces / collide
   LaunchFn mom_init = nullptr;  // f = f_eq
@@ -16,6 +21,7 @@ struct Launchers {
   LaunchFn aa_step = nullptr;   // in-place AA-pattern step (momentum only)
   LaunchFn aa_init = nullptr;
   LaunchFn aa_readback = nullptr;
+  ServeLaunchFn serve[2] = {nullptr, nullptr};  // BGK, TRT (D2Q9 / D3Q19, AA storages)
 };
 
 // precision: LBF_F64 .. LBF_BF16_SHIFTED; returns false when not built
@@ -32,6 +38,18 @@ hipError_t launch_advance(const KArgs& a, long long nblocks, hipStream_t s);
 hipError_t launch_reduce(const void* values, int is_double, const uint32_t* cls, int need_link,
                          long long N, double* partials, int grid, double* out, int count_nonfinite,
                          hipStream_t s);
+
+struct ClassArgs {
+  int nx, ny, nz, q;
+  long long N, ld;
+  int c[27][3];
+};
+hipError_t launch_build_classes(const ClassArgs& a, const uint8_t* type, uint32_t* cls,
+                                hipStream_t s);
+hipError_t launch_fill(void* p, int is_double, long long n_valid, long long n_total, double value,
+                       hipStream_t s);
+hipError_t launch_qads_init(const uint32_t* cls, double* q, long long ld, double q0,
+                            hipStream_t s);
 
 hipError_t launch_nonfinite(const void* rho, const void* u, int is_double, int d, long long ld,
                             const uint32_t* cls, long long N, double* partials, int grid,

@@ -248,23 +248,18 @@ __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
   });
 }
 
-// AA-pattern fused step; OP 0 BGK, 1 TRT, 2 MRT; ODD selects the step parity.
-// MOM: this launch may store moments / residual partials (the last step of a
-// batch, or steps with residuals); the streaming steps in between use the
-// MOM=false instantiation, which is compiled for full occupancy.
+// One node of the AA-pattern fused step (OP 0 BGK, 1 TRT, 2 MRT; ODD selects
+// the step parity). `step` is the case's completed-step count, `fresh` marks
+// an array that holds post-stream f in natural slots. MOM: store moments (and
+// accumulate residual terms into acc) as the flags ask.
 template <class L, class St, int OP, bool ODD, bool MOM>
-__global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(MOM ? 2 : (sizeof(typename St::T) == 8 ? 4 : (ODD ? 6 : 8)))))
-    momentum_step_aa(KArgs a) {
+__device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned long long step,
+                                        bool fresh, double* acc) {
 #pragma clang fp contract(fast)
   using T = typename St::T;
   using S = typename St::S;
   using k = K<L>;
   constexpr int Q = L::Q, D = L::D;
-  Ctrl* ctrl = a.ctrl;
-  if (ctrl->stopped) return;
-  const unsigned long long step = ctrl->step;
-  const bool fresh = a.fresh != 0;
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(a.pop[0], (short)0, (int)0xffffffff, 0x00020000);
   const uint32_t pb = a.plane_bytes;
@@ -280,14 +275,6 @@ __global__ void __launch_bounds__(kBlock)
   const T sc = T(a.scale), isc = T(a.inv_scale);
   const uint32_t N = (uint32_t)a.N;
   const uint32_t ld = (uint32_t)a.ld;
-  double acc[5] = {0, 0, 0, 0, 0};
-  // one node per thread (the host launches ceil(N / kBlock) workgroups).
-  // xcd_slab: workgroups are dealt round-robin to the 8 XCDs, so workgroup b
-  // takes node block (b % 8) * (grid / 8) + b / 8 and each XCD sweeps one
-  // contiguous slab; y neighbours then share that XCD's L2 (host: grid % 8 == 0).
-  uint32_t blk = blockIdx.x;
-  if (a.xcd_slab) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
-  const uint32_t n = blk * kBlock + threadIdx.x;
   const uint32_t cls = n < N ? a.cls[n] : kSolidBit;
   if (!(cls & kSolidBit)) {
     NodePos p;
@@ -383,6 +370,28 @@ __global__ void __launch_bounds__(kBlock)
       }
     });
   }
+}
+
+// AA-pattern fused step over one case. MOM: this launch may store moments /
+// residual partials (the last step of a batch, or steps with residuals); the
+// streaming steps in between use the MOM=false instantiation, which is
+// compiled for full occupancy.
+template <class L, class St, int OP, bool ODD, bool MOM>
+__global__ void __launch_bounds__(kBlock)
+    __attribute__((amdgpu_waves_per_eu(MOM ? 2 : (sizeof(typename St::T) == 8 ? 4 : (ODD ? 6 : 8)))))
+    momentum_step_aa(KArgs a) {
+  Ctrl* ctrl = a.ctrl;
+  if (ctrl->stopped) return;
+  const unsigned long long step = ctrl->step;
+  double acc[5] = {0, 0, 0, 0, 0};
+  // one node per thread (the host launches ceil(N / kBlock) workgroups).
+  // xcd_slab: workgroups are dealt round-robin to the 8 XCDs, so workgroup b
+  // takes node block (b % 8) * (grid / 8) + b / 8 and each XCD sweeps one
+  // contiguous slab; y neighbours then share that XCD's L2 (host: grid % 8 == 0).
+  uint32_t blk = blockIdx.x;
+  if (a.xcd_slab) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
+  const uint32_t n = blk * kBlock + threadIdx.x;
+  aa_node<L, St, OP, ODD, MOM>(a, n, step, a.fresh != 0, acc);
   if (a.flags & kFlagResiduals) {
     __shared__ double lds[kBlock / 64];
     for (int kk = 0; kk < 4; kk++) {
@@ -393,6 +402,7 @@ __global__ void __launch_bounds__(kBlock)
     if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + 8] = s;
   }
 }
+
 
 // Post-stream populations (f, not delta) into out[q][ld] as f64: the first
 // half of the next step, for read-back.

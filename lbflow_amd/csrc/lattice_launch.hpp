@@ -4,6 +4,7 @@
 #include "dispatch.hpp"
 #include "fast.hpp"
 #include "kernels.hpp"
+#include "serve.hpp"
 
 namespace lbf {
 
@@ -63,6 +64,15 @@ static hipError_t aa_init_t(const KArgs& a, int grid, hipStream_t s) {
   return launch(momentum_init_aa<L, St>, a, grid, s);
 }
 
+// serving-tier batched steps exist for the lattices of the serving mix
+template <class L, class St>
+static void serve_launchers(Launchers* out) {
+  if constexpr (L::id == 1 || L::id == 3) {
+    out->serve[0] = serve_launch<L, St, 0>;
+    out->serve[1] = serve_launch<L, St, 1>;
+  }
+}
+
 template <class L>
 static bool make_launchers(int precision, Launchers* out) {
   switch (precision) {
@@ -83,21 +93,25 @@ static bool make_launchers(int precision, Launchers* out) {
       out->aa_step = aa_step_t<L, StF32>;
       out->aa_init = aa_init_t<L, StF32>;
       out->aa_readback = aa_readback_t<L, StF32>;
+      serve_launchers<L, StF32>(out);
       return true;
     case 2:  // LBF_F32_SHIFTED
       out->aa_step = aa_step_t<L, StF32S>;
       out->aa_init = aa_init_t<L, StF32S>;
       out->aa_readback = aa_readback_t<L, StF32S>;
+      serve_launchers<L, StF32S>(out);
       return true;
     case 3:  // LBF_F16_SHIFTED
       out->aa_step = aa_step_t<L, StF16S>;
       out->aa_init = aa_init_t<L, StF16S>;
       out->aa_readback = aa_readback_t<L, StF16S>;
+      serve_launchers<L, StF16S>(out);
       return true;
     case 4:  // LBF_BF16_SHIFTED
       out->aa_step = aa_step_t<L, StBF16S>;
       out->aa_init = aa_init_t<L, StBF16S>;
       out->aa_readback = aa_readback_t<L, StBF16S>;
+      serve_launchers<L, StBF16S>(out);
       return true;
     default:
       return false;

@@ -123,6 +123,83 @@ __global__ void __launch_bounds__(kBlock) copy_kernel(const uint4* __restrict__ 
   if (i < n) dst[i] = src[i];
 }
 
+// ---- case set-up kernels ---------------------------------------------------
+
+// Node classes (src/momentum/lattice.rs:127-191): for a fluid node, bit k is
+// set when its neighbour in direction k, wrapped periodically on every axis
+// as the reference wraps it (lattice.rs:139), is solid; solid nodes and the
+// padding up to ld get kSolidBit. type == nullptr: every node is fluid.
+__global__ void __launch_bounds__(kBlock) build_classes_kernel(ClassArgs a, const uint8_t* type,
+                                                               uint32_t* cls) {
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.ld) return;
+  if (i >= a.N || (type && type[i])) {
+    cls[i] = kSolidBit;
+    return;
+  }
+  uint32_t m = 0;
+  if (type) {
+    const long long nxy = (long long)a.nx * a.ny;
+    const int z = (int)(i / nxy);
+    const int y = (int)((i - (long long)z * nxy) / a.nx);
+    const int x = (int)(i - (long long)z * nxy - (long long)y * a.nx);
+    for (int k = 0; k < a.q; k++) {
+      int xx = x + a.c[k][0], yy = y + a.c[k][1], zz = z + a.c[k][2];
+      xx = xx < 0 ? xx + a.nx : (xx >= a.nx ? xx - a.nx : xx);
+      yy = yy < 0 ? yy + a.ny : (yy >= a.ny ? yy - a.ny : yy);
+      zz = zz < 0 ? zz + a.nz : (zz >= a.nz ? zz - a.nz : zz);
+      if (type[xx + a.nx * ((long long)yy + (long long)a.ny * zz)]) m |= 1u << k;
+    }
+  }
+  cls[i] = m;
+}
+
+hipError_t launch_build_classes(const ClassArgs& a, const uint8_t* type, uint32_t* cls,
+                                hipStream_t s) {
+  const long long blocks = (a.ld + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(build_classes_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, a, type,
+                     cls);
+  return hipGetLastError();
+}
+
+// p[i] = value for i < n_valid, 0 for n_valid <= i < n_total
+template <class T>
+__global__ void __launch_bounds__(kBlock) fill_kernel(T* p, long long n_valid, long long n_total,
+                                                      double value) {
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n_total) p[i] = i < n_valid ? (T)value : (T)0;
+}
+
+hipError_t launch_fill(void* p, int is_double, long long n_valid, long long n_total, double value,
+                       hipStream_t s) {
+  const long long blocks = (n_total + kBlock - 1) / kBlock;
+  if (blocks == 0) return hipSuccess;
+  if (is_double)
+    hipLaunchKernelGGL(fill_kernel<double>, dim3((unsigned)blocks), dim3(kBlock), 0, s,
+                       (double*)p, n_valid, n_total, value);
+  else
+    hipLaunchKernelGGL(fill_kernel<float>, dim3((unsigned)blocks), dim3(kBlock), 0, s, (float*)p,
+                       n_valid, n_total, value);
+  return hipGetLastError();
+}
+
+// q_ads initial value on momentum bounce-back nodes (fluid nodes with a solid
+// neighbour), 0 elsewhere (src/passive_scalar/node.rs:77-85)
+__global__ void __launch_bounds__(kBlock) qads_init_kernel(const uint32_t* cls, double* q,
+                                                           long long ld, double q0) {
+  const long long i = (long long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ld) return;
+  const uint32_t c = cls[i];
+  q[i] = (!(c & kSolidBit) && (c & kLinkMask)) ? q0 : 0.0;
+}
+
+hipError_t launch_qads_init(const uint32_t* cls, double* q, long long ld, double q0,
+                            hipStream_t s) {
+  const long long blocks = (ld + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL(qads_init_kernel, dim3((unsigned)blocks), dim3(kBlock), 0, s, cls, q, ld, q0);
+  return hipGetLastError();
+}
+
 }  // namespace lbf
 
 // Benchmark hook (not part of the reference-equivalent ABI): copies `bytes`

@@ -41,6 +41,7 @@ def load_library(path: Optional[str] = None):
             f"HIP engine library not found at {p}; build it with `python -m lbflow_amd.build`")
     lib = C.CDLL(p)
     abi.declare(lib, abi.ENTRY_POINTS)
+    abi.declare(lib, abi.SERVER_ENTRY_POINTS)
     if path is None:
         _lib = lib
     return lib

@@ -1,0 +1,122 @@
+"""Serving tier on the GPU: a served case equals the single-case path, each
+result is bitwise the same alone and inside a mix, admission keeps the
+resident bytes within the HBM budget, and unservable cases are refused."""
+import numpy as np
+import pytest
+
+from lbflow_amd import abi, configs
+from lbflow_amd.engine import Case, LbfError
+from lbflow_amd.serve import Server
+
+pytestmark = pytest.mark.gpu
+
+
+def case(vs, side, prec, seed, tau=0.8, porosity=0.8):
+    return configs.serving_case(vs, side, tau=tau, porosity=porosity, seed=seed, precision=prec)
+
+
+def single(spec, steps):
+    g = Case(spec)
+    g.initialize_nodes()
+    g.main_steps(steps)
+    rho, u = g.fields()
+    pops = g.pops()
+    g.close()
+    return rho, u, pops
+
+
+@pytest.mark.parametrize("vs,side,prec,steps", [
+    (abi.D2Q9, 40, abi.F32, 37),
+    (abi.D3Q19, 20, abi.F16_SHIFTED, 21),
+    (abi.D3Q19, 18, abi.BF16_SHIFTED, 8),
+])
+def test_served_case_equals_single_case_path(vs, side, prec, steps):
+    spec = case(vs, side, prec, seed=3)
+    print(f"[serve-single] vs={vs} side={side} prec={prec} steps={steps}", flush=True)
+    r0, u0, p0 = single(spec, steps)
+    with Server(max_steps_per_round=16) as srv:
+        t = srv.submit(spec, steps, keep_pops=True)
+        srv.wait(t, 120)
+        r1, u1, p1 = srv.result(t)
+    assert np.array_equal(p0, p1)
+    assert np.array_equal(r0, r1) and np.array_equal(u0, u1)
+
+
+def test_result_is_bitwise_the_same_alone_and_in_a_mix():
+    target = case(abi.D3Q19, 24, abi.F32, seed=11, tau=0.7)
+    steps = 45
+    with Server() as srv:
+        t = srv.submit(target, steps, keep_pops=True)
+        srv.wait(t, 120)
+        alone = srv.result(t)
+    rng = np.random.default_rng(1)
+    print("[serve-mix] submitting mix", flush=True)
+    with Server(max_steps_per_round=8) as srv:
+        others = []
+        for i in range(10):
+            vs = abi.D2Q9 if i % 2 else abi.D3Q19
+            prec = (abi.F32, abi.F16_SHIFTED, abi.BF16_SHIFTED, abi.F32_SHIFTED)[i % 4]
+            side = int(rng.integers(16, 48)) if vs == abi.D2Q9 else int(rng.integers(8, 20))
+            sp = case(vs, side, prec, seed=100 + i, tau=float(rng.uniform(0.55, 1.2)))
+            others.append((srv.submit(sp, int(rng.integers(5, 80))), sp))
+            if i == 4:
+                t = srv.submit(target, steps, keep_pops=True)  # joins while others run
+        srv.drain(300)
+        st = srv.stats()
+        print(f"[serve-mix] stats {st}", flush=True)
+        mixed = srv.result(t)
+        assert st["completed"] == 11 and st["failed"] == 0
+        assert st["graphs_built"] >= 1 and st["rounds"] >= 1
+        for ticket, sp in others:
+            rho, u, _ = srv.result(ticket)
+            assert np.isfinite(u).all()
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a, b)
+
+
+def test_admission_respects_the_hbm_budget():
+    spec = case(abi.D3Q19, 24, abi.F32, seed=5)
+    from lbflow_amd.engine import device_bytes
+    nb = device_bytes(spec)
+    budget = int(2.5 * nb)
+    with Server(hbm_budget_bytes=budget, max_steps_per_round=4) as srv:
+        ts = [srv.submit(case(abi.D3Q19, 24, abi.F32, seed=5 + i), 30) for i in range(6)]
+        srv.drain(300)
+        st = srv.stats()
+        assert st["completed"] == 6
+        assert st["hbm_peak"] <= budget
+        infos = [srv.info(t) for t in ts]
+    # at most two cases resident at any time
+    events = sorted([(i.t_admit, 1) for i in infos] + [(i.t_done, -1) for i in infos])
+    live = peak = 0
+    for _, d in events:
+        live += d
+        peak = max(peak, live)
+    assert peak <= 2
+
+
+def test_unservable_cases_are_refused():
+    with Server() as srv:
+        refused = [
+            configs.porous_channel(n=8, precision=abi.F64),             # FP64 parity path
+            configs.coupled_discs(nx=32, ny=16, count=3),               # scalar + Zou-He
+            configs.porous_channel(n=8, precision=abi.F32,
+                                   op=configs.Collision.mrt([1.0] * 19)),  # MRT
+            configs.porous_channel(n=8, precision=abi.F32, velocity_set=abi.D3Q27),
+        ]
+        for sp in refused:
+            with pytest.raises(LbfError) as e:
+                srv.submit(sp, 10)
+            assert e.value.code == abi.EUNSUPPORTED
+        t = srv.submit(configs.cavity(n=16, precision=abi.F32), 10)  # walls + moving lid
+        srv.wait(t, 60)
+
+
+def test_zero_steps_returns_the_initial_fields():
+    spec = case(abi.D2Q9, 16, abi.F32, seed=2)
+    with Server() as srv:
+        t = srv.submit(spec, 0)
+        srv.wait(t, 60)
+        rho, u, _ = srv.result(t)
+    fluid = ~spec.node_type.astype(bool).reshape(-1)
+    assert np.allclose(rho[fluid], 1.0) and np.allclose(u[fluid], 0.0)
