@@ -258,6 +258,85 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
     }
 
 
+def precision_leg(scale=1.0):
+    """Precision tier per configuration: relative L2 velocity error and global
+    mass drift of each storage against the FP64 parity path after the
+    configuration's full step count (x scale), plus wall time.
+
+      cavity   D2Q9 BGK lid-driven cavity 128^2, tau 0.6, lid 0.1, 20,000 steps
+      porous   D3Q19 TRT porous channel 256^3, 5,000 steps
+      sphere   D3Q27 MRT (this build's basis) flow past a sphere, 384x128x128,
+               periodic x, walls y/z, initial u = 0.05, 10,000 steps
+      discs    D2Q9 BGK + D2Q9 scalar with adsorption, 1024x512, 200 discs,
+               Zou-He inlet/outlet, 50,000 steps (FP32 two-array vs FP64)
+    """
+    import numpy as np
+    from lbflow_amd import abi, configs
+    from lbflow_amd.case import Collision
+    from lbflow_amd.engine import Case
+
+    def run(spec, steps, scalar=False):
+        t0 = time.perf_counter()
+        c = Case(spec)
+        c.initialize_nodes()
+        done = 0
+        while done < steps:  # batches keep the host loop short
+            k = min(1000, steps - done)
+            c.main_steps(k)
+            done += k
+        c.synchronize()
+        wall = time.perf_counter() - t0
+        rho, u = c.fields()
+        mass = c.reduce(abi.REDUCE_TOTAL_MASS)
+        out = dict(u=u, rho=rho, mass=mass, wall=wall)
+        if scalar:
+            cv, qa = c.scalar(0)
+            out.update(c=cv, ads=c.reduce(abi.REDUCE_ADSORBED),
+                       ctot=c.reduce(abi.REDUCE_SCALAR_TOTAL))
+        c.close()
+        return out
+
+    def l2rel(a, b, fluid):
+        a, b = a[fluid], b[fluid]
+        return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+    tau = 0.6
+    mrt27 = Collision.mrt([1.0 / tau] * 10 + [1.2] * 17)
+    cases = [
+        ("cavity", lambda p: configs.cavity(n=128, tau=0.6, lid=0.1, precision=p), 20000,
+         [abi.F32, abi.F32_SHIFTED, abi.F16_SHIFTED, abi.BF16_SHIFTED]),
+        ("porous", lambda p: configs.porous_channel(n=256, precision=p), 5000,
+         [abi.F32, abi.F32_SHIFTED, abi.F16_SHIFTED, abi.BF16_SHIFTED]),
+        ("sphere", lambda p: configs.sphere_channel(precision=p, op=mrt27), 10000,
+         [abi.F32, abi.F16_SHIFTED, abi.BF16_SHIFTED]),
+        ("discs", lambda p: configs.coupled_discs(precision=p), 50000, [abi.F32]),
+    ]
+    report = {}
+    for name, make, steps, precs in cases:
+        steps = max(1, int(steps * scale))
+        print(f"[precision] {name}: f64 reference, {steps} steps", file=sys.stderr, flush=True)
+        ref_spec = make(abi.F64)
+        fluid = ~ref_spec.node_type.astype(bool).reshape(-1) if ref_spec.node_type is not None \
+            else np.ones(ref_spec.num_nodes, bool)
+        scalar = bool(ref_spec.scalars)
+        ref = run(ref_spec, steps, scalar)
+        entry = {"steps": steps, "nodes": ref_spec.num_nodes, "f64_wall_s": round(ref["wall"], 3)}
+        for p in precs:
+            name_p = abi.PRECISION_NAMES[p]
+            print(f"[precision] {name}: {name_p}", file=sys.stderr, flush=True)
+            r = run(make(p), steps, scalar)
+            e = {"l2_u_rel": l2rel(r["u"], ref["u"], fluid),
+                 "mass_drift_rel": abs(r["mass"] - ref["mass"]) / abs(ref["mass"]),
+                 "wall_s": round(r["wall"], 3),
+                 "nonfinite": bool(not np.isfinite(r["u"][fluid]).all())}
+            if scalar:
+                e["l2_c_rel"] = l2rel(r["c"], ref["c"], fluid)
+                e["adsorbed_rel"] = abs(r["ads"] - ref["ads"]) / max(abs(ref["ads"]), 1e-300)
+            entry[name_p] = e
+        report[name] = entry
+    return report
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -268,12 +347,18 @@ def main():
     ap.add_argument("--only", default="", help="comma list of variants (f32,f64,f16s,...)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--serve-cases", type=int, default=2000)
+    ap.add_argument("--precision", action="store_true",
+                    help="only the precision-tier report (errors vs FP64 per configuration)")
+    ap.add_argument("--precision-scale", type=float, default=1.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
               f"(every case fits one MI355X's HBM; multi-GPU sharding is not built)", file=sys.stderr)
         sys.exit(2)
 
+    if args.precision:
+        print(json.dumps({"precision": precision_leg(args.precision_scale)}))
+        return
     from lbflow_amd import abi, configs
     hip = Hip()
     n = args.n

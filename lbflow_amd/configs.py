@@ -69,8 +69,9 @@ def coupled_discs(nx=1024, ny=512, count=200, rmin=4.0, rmax=12.0, seed=4, tau=0
 
 def sphere_channel(nx=384, ny=128, nz=128, radius=16, u_in=0.05, tau=0.6, precision=abi.F64,
                    velocity_set=abi.D3Q27, op=None) -> Spec:
-    """Flow past a sphere (bounce-back), periodic lateral faces, velocity given
-    by a uniform body force so every face stays reference-constructible."""
+    """Flow past a bounce-back sphere: periodic along x, no-slip walls on y/z,
+    uniform initial velocity u_in along x (no inlet/outlet: the reference has
+    no 3-D Zou-He closure, so the flow decays from its initial state)."""
     mask = sphere_mask(nx, ny, nz, (nx // 4, ny // 2, nz // 2), radius)
     return Spec(
         velocity_set=velocity_set, n=(nx, ny, nz), collision=op or Collision.bgk(tau),
