@@ -173,6 +173,14 @@ typedef struct lbf_case_desc {
   int32_t streaming;  /* LBF_STREAM_* */
   const lbf_scalar_desc* scalars;
   double shift_scale; /* shifted storage: stored = (f - w)*scale; 0 = automatic */
+  /* Shan-Chen pseudo-potential force (`multiphase::Parameters`,
+   * src/momentum/multiphase.rs:4-94): F = -G psi(rho) sum_i w_i c_i psi_i with
+   * psi(rho) = 1 - exp(-rho) (src/kernel.rs:171-174) and psi(wall_density) for
+   * solid neighbours; added to the body force (node.rs:388-404). */
+  int32_t multiphase; /* 0 = off */
+  int32_t reserved;
+  double g_function;   /* G */
+  double wall_density; /* rho_w */
 } lbf_case_desc;
 
 typedef struct lbf_case lbf_case;

@@ -91,7 +91,8 @@ class CaseDesc(C.Structure):
                 ("initial_density_uniform", C.c_double), ("initial_velocity", _dp),
                 ("initial_velocity_uniform", C.c_double * 3), ("n_scalars", C.c_uint32),
                 ("streaming", C.c_int32), ("scalars", C.POINTER(ScalarDesc)),
-                ("shift_scale", C.c_double)]
+                ("shift_scale", C.c_double), ("multiphase", C.c_int32), ("reserved", C.c_int32),
+                ("g_function", C.c_double), ("wall_density", C.c_double)]
 
 
 # C-ABI entry points of the product library: (name, restype, argtypes).

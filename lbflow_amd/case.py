@@ -140,6 +140,7 @@ class Spec:
     streaming: int = abi.STREAM_AUTO
     shift_scale: float = 0.0
     freeze_momentum: bool = False
+    multiphase: object = None                # None or (G, wall_density): Shan-Chen force
 
     @property
     def d(self) -> int:
@@ -216,6 +217,10 @@ class Spec:
         d.precision = self.precision
         d.streaming = self.streaming
         d.shift_scale = float(self.shift_scale)
+        if self.multiphase is not None:
+            d.multiphase = 1
+            d.g_function = float(self.multiphase[0])
+            d.wall_density = float(self.multiphase[1])
         d.n_scalars = len(self.scalars)
         if self.scalars:
             arr = (abi.ScalarDesc * len(self.scalars))()

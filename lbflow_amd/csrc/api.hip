@@ -89,7 +89,7 @@ bool vs_info(int id, VsInfo* v) {
 // slots, which reproduces no-slip and moving-wall bounce-back exactly; it
 // needs opposite faces to be both periodic or both not, and no scalars.
 bool aa_compatible(const lbf_case_desc* d, const VsInfo& v) {
-  if (d->n_scalars) return false;
+  if (d->n_scalars || d->multiphase) return false;
   for (int f = 0; f < 2 * v.d; f++) {
     const int k = d->faces[f].kind;
     if (k != LBF_BC_NO_SLIP && k != LBF_BC_BOUNCE_BACK && k != LBF_BC_PERIODIC) return false;
@@ -181,6 +181,7 @@ struct lbf_case {
   void* rho[2] = {nullptr, nullptr};
   void* u[2] = {nullptr, nullptr};
   double* force = nullptr;
+  void* phi = nullptr;  // Shan-Chen pseudo-potential of the current step
   Ctrl* ctrl = nullptr;
   double* partials = nullptr;
   double* readback = nullptr;
@@ -244,7 +245,7 @@ void free_case(lbf_case* c) {
       (void)hipFree(p);
   };
   F(c->pop[0]), F(c->pop[1]), F(c->cls), F(c->rho[0]), F(c->rho[1]), F(c->u[0]), F(c->u[1]),
-      F(c->force), F(c->ctrl), F(c->partials), F(c->readback), F(c->red_out);
+      F(c->force), F(c->ctrl), F(c->partials), F(c->readback), F(c->red_out), F(c->phi);
   for (int s = 0; s < c->ns; s++) {
     ScalarDev& S = c->sc[s];
     F(S.pop[0]), F(S.pop[1]), F(S.c[0]), F(S.c[1]), F(S.q_ads), F(S.src);
@@ -386,6 +387,7 @@ int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
                   (d->streaming == LBF_STREAM_AUTO && aa_ok && d->precision != LBF_F64);
   uint64_t b = (aa ? 1ull : 2ull) * v.q * ld * sb + 4ull * ld + 2ull * (1 + v.d) * ld * tb;
   if (d->force_kind == LBF_SOURCE_FIELD) b += 8ull * v.d * ld;
+  if (d->multiphase) b += (uint64_t)ld * tb;
   for (uint32_t s = 0; s < d->n_scalars; s++) {
     b += 2ull * v.q * ld * tb + 2ull * ld * tb;
     if (d->scalars[s].has_adsorption) b += 8ull * ld;
@@ -528,6 +530,7 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   if (launch_ctrl_set(c->ctrl, kCtrlMaxIter, -1, c->stream) != hipSuccess)  // max_iter = ~0
     return cleanup(fail(LBF_EDEVICE, "control block initialisation failed"));
   if ((rc = alloc((void**)&c->partials, (size_t)c->grid * kMaxCols * 8))) return cleanup(rc);
+  if (d->multiphase && (rc = alloc(&c->phi, c->ld * c->tb))) return cleanup(rc);
   if ((rc = alloc((void**)&c->red_out, 64))) return cleanup(rc);
 
   // initial moments into both buffers (solid nodes keep them forever)
@@ -609,6 +612,12 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   a.u[0] = c->u[0], a.u[1] = c->u[1];
   a.partials = c->partials;
   a.ctrl = c->ctrl;
+  if (d->multiphase) {
+    a.multiphase = 1;
+    a.sc_g = d->g_function;
+    a.wall_phi = 1.0 * (1.0 - std::exp(-d->wall_density / 1.0));  // multiphase.rs:10-18
+    a.phi = c->phi;
+  }
   for (int f = 0; f < 6; f++) {
     a.face[f].kind = f < nf ? d->faces[f].kind : LBF_BC_PERIODIC;
     a.face[f].known = d->faces[f].known;
@@ -706,10 +715,17 @@ int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
     a.flags = kf & ~kFlagStopCheck;
     if (resid || last_of_batch || c->needs_moments) a.flags |= kFlagStoreMoments;
     a.res_mask = mask;
-    if (c->aa)
+    if (c->aa) {
       HIPCHK(c->L.aa_step(a, c->grid, c->stream));
-    else
+    } else {
+      if (c->desc.multiphase) {
+        // psi of every node of this step before any node's force reads it
+        KArgs p = a;
+        p.mode = kModePhi;
+        HIPCHK(c->L.mom_step(p, c->grid, c->stream));
+      }
       HIPCHK(c->L.mom_step(a, c->grid, c->stream));
+    }
   }
   for (int s = 0; s < c->ns; s++) {
     KArgs a = scalar_args(c, s);

@@ -13,7 +13,7 @@ constexpr uint32_t kLinkMask = (1u << 27) - 1;  // bits 0..26: neighbour i is so
 constexpr int kMaxCols = 12;  // residual columns: rho, u0..u2, c0..c3, nonfinite, spare
 constexpr int kHist = 64;     // residual history ring
 
-enum Mode : int { kModePull = 0, kModeStreamOnly = 1 };
+enum Mode : int { kModePull = 0, kModeStreamOnly = 1, kModePhi = 2 };
 enum Flag : uint32_t {
   kFlagStoreMoments = 1u,
   kFlagResiduals = 2u,
@@ -85,6 +85,10 @@ struct KArgs {
   FaceK face[6];   // momentum face table (also used by scalars for the wall velocity)
   SFaceK sface[6];
   double scale, inv_scale;  // shifted storage
+  // Shan-Chen force: psi of every fluid node of the current step in phi[ld]
+  int multiphase;
+  double sc_g, wall_phi;
+  void* phi;
 };
 
 }  // namespace lbf

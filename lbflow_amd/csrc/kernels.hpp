@@ -234,10 +234,11 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
   using k = K<L>;
   constexpr int Q = L::Q, D = L::D;
   Ctrl* ctrl = a.ctrl;
-  if (a.mode == kModePull && ctrl->stopped) return;
+  if (a.mode != kModeStreamOnly && ctrl->stopped) return;
   const unsigned long long step = ctrl->step;
   const int par = (int)(step & 1);
   const bool fresh = ctrl->fresh[0] != 0;
+  T* phi = (T*)a.phi;
   const S* __restrict__ src = (const S*)a.pop[par];
   S* __restrict__ dst = (S*)a.pop[par ^ 1];
   double* __restrict__ outd = (double*)a.out;
@@ -284,11 +285,46 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
     }
     T rho, u[3];
     moments<L>(f, rho, u);
+    if (a.mode == kModePhi) {
+      // compute_phi (src/momentum/multiphase.rs:48-54): psi = 1 - exp(-rho)
+      phi[n] = T(1) * (T(1) - exp(-rho / T(1)));
+      continue;
+    }
     T F[3] = {T(0), T(0), T(0)};
-    if (a.force_kind) {
+    const bool has_force = a.force_kind || a.multiphase;
+    if (has_force) {
+      // body force (+ Shan-Chen force, node.rs:388-404; multiphase.rs:56-93)
+      T B[3] = {T(0), T(0), T(0)};
+      if (a.force_kind)
+        sfor<D>([&](auto A) {
+          constexpr int ax = decltype(A)::value;
+          B[ax] = a.force_kind == 1 ? T(a.F[ax]) : T(a.src_field[ax * ld + n]);
+        });
+      if (a.multiphase) {
+        Geo g;
+        g.init(n, a.nx, a.ny, a.nz);
+        T ph[Q];
+        sfor<Q>([&](auto I) {
+          constexpr int i = decltype(I)::value;
+          ph[i] = (cls >> i & 1u) ? T(a.wall_phi)
+                                  : phi[n + g.off(k::cx(i), k::cy(i), k::cz(i))];
+        });
+        const T node_phi = phi[n];
+        sfor<D>([&](auto A) {
+          constexpr int ax = decltype(A)::value;
+          T sum = T(0);
+          sfor<Q>([&](auto I) {
+            constexpr int i = decltype(I)::value;
+            sum = sum + T(k::w(i)) * T(k::ca(i, ax)) * ph[i];
+          });
+          const T sc = -T(a.sc_g) * node_phi * sum;
+          F[ax] = a.force_kind ? B[ax] + sc : sc;
+        });
+      } else {
+        sfor<D>([&](auto A) { F[decltype(A)::value] = B[decltype(A)::value]; });
+      }
       sfor<D>([&](auto A) {
         constexpr int ax = decltype(A)::value;
-        F[ax] = a.force_kind == 1 ? T(a.F[ax]) : T(a.src_field[ax * ld + n]);
         u[ax] += T(0.5) * F[ax] / rho;  // node.rs:450-455
       });
     }
@@ -315,7 +351,7 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
     equilibrium<L>(rho, u, feq);
     if constexpr (OP == 0) {
       bgk<L>(f, feq, omega, omega_p, fs);
-      if (a.force_kind) add_guo<L>(u, F, coeff_b, fs);
+      if (has_force) add_guo<L>(u, F, coeff_b, fs);
     } else if constexpr (OP == 1) {
       trt<L>(f, feq, wp, wm, fs);
     } else {
@@ -331,7 +367,7 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
       stp<St>(dst + j * ld + n, fs[j]);
     });
   }
-  if (a.mode == kModeStreamOnly) return;
+  if (a.mode != kModePull) return;
   // residual partials: columns 0 rho, 1..3 u, 8 non-finite count
   if (a.flags & kFlagResiduals) {
     __shared__ double lds[kBlock / 64];

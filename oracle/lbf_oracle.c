@@ -474,7 +474,10 @@ struct lbo_case {
   long nbb;
   unsigned* bbmask; /* [N] bit i: neighbour i solid (lattice.rs:177-191) */
   double *f, *feq, *fs, *rho, *u, *rho_sh, *u_sh, *force_field;
-  int has_force;
+  int has_force;    /* a body force (uniform or field) is given */
+  int multiphase;   /* Shan-Chen force on (src/momentum/multiphase.rs) */
+  double sc_g, wall_phi;
+  double* phi;      /* [N] pseudo-potential of the current step */
   scalar_lat sc[LBF_MAX_SCALARS];
   int ns;
   double res_rho, res_u[3];
@@ -483,10 +486,52 @@ struct lbo_case {
 };
 typedef struct lbo_case lbo_case;
 
-static void get_force(const lbo_case* L, long n, double* F) {
-  for (int x = 0; x < L->d; x++)
-    F[x] = L->desc.force_kind == LBF_SOURCE_FIELD ? L->force_field[n * L->d + x]
-                                                  : L->desc.force[x];
+/* kernel::pseudo_potential, src/kernel.rs:171-174 (LATTICE_DENSITY = 1) */
+static double pseudo_potential(double density) {
+  const double ref = 1.0;
+  return ref * (1.0 - exp(-density / ref));
+}
+
+/* Node::compute_shan_chen_force, src/momentum/multiphase.rs:56-93: every
+ * direction's neighbour (periodic wrap on every axis) contributes its psi,
+ * a solid neighbour the wall's. The reference computes this force twice per
+ * step: in compute_velocity, inside the same parallel loop that computes
+ * each node's density and psi (so a neighbour's psi may still be the
+ * previous step's, depending on thread timing), and in the BGK collision,
+ * after every psi is current. This restatement uses the current psi in both
+ * places, which is the collision's value and one of the orders the
+ * velocity pass can see. */
+static void shan_chen_force(const lbo_case* L, long n, double* F) {
+  const int q = L->q, D = L->d;
+  const double node_phi = L->phi[n];
+  for (int x = 0; x < D; x++) {
+    double sum = 0.0;
+    for (int i = 0; i < q; i++) {
+      long nb = L->nbr[n * q + i];
+      double phi_i = L->type[nb] ? L->wall_phi : L->phi[nb];
+      sum = sum + L->v.w[i] * (double)L->v.c[i][x] * phi_i;
+    }
+    F[x] = -L->sc_g * node_phi * sum;
+  }
+}
+
+/* Node::get_force, src/momentum/node.rs:388-404: body force, Shan-Chen
+ * force, or their sum. Returns 0 when the node has no force. */
+static int get_force(const lbo_case* L, long n, double* F) {
+  if (!L->has_force && !L->multiphase) return 0;
+  double B[3] = {0, 0, 0};
+  if (L->has_force)
+    for (int x = 0; x < L->d; x++)
+      B[x] = L->desc.force_kind == LBF_SOURCE_FIELD ? L->force_field[n * L->d + x]
+                                                    : L->desc.force[x];
+  if (!L->multiphase) {
+    for (int x = 0; x < L->d; x++) F[x] = B[x];
+    return 1;
+  }
+  double S[3];
+  shan_chen_force(L, n, S);
+  for (int x = 0; x < L->d; x++) F[x] = L->has_force ? B[x] + S[x] : S[x];
+  return 1;
 }
 
 void lbo_destroy(lbo_case* L) {
@@ -494,7 +539,7 @@ void lbo_destroy(lbo_case* L) {
   free(L->type), free(L->nbr), free(L->fluid), free(L->bb), free(L->bbmask);
   for (int f = 0; f < 6; f++) free(L->face[f]);
   free(L->f), free(L->feq), free(L->fs), free(L->rho), free(L->u), free(L->rho_sh),
-      free(L->u_sh), free(L->force_field);
+      free(L->u_sh), free(L->force_field), free(L->phi);
   for (int s = 0; s < L->ns; s++) {
     scalar_lat* S = &L->sc[s];
     free(S->g), free(S->geq), free(S->gs), free(S->c), free(S->c_shallow), free(S->q_ads),
@@ -620,6 +665,13 @@ int lbo_create(const lbf_case_desc* d, lbo_case** out) {
   memcpy(L->rho_sh, L->rho, sizeof(double) * N);
   memcpy(L->u_sh, L->u, sizeof(double) * N * D);
   L->has_force = d->force_kind != LBF_SOURCE_NONE;
+  L->multiphase = d->multiphase != 0;
+  if (L->multiphase) {
+    L->sc_g = d->g_function;
+    L->wall_phi = pseudo_potential(d->wall_density); /* multiphase.rs:10-18 */
+    L->phi = (double*)calloc(N, sizeof(double));
+    for (long i = 0; i < N; i++) L->phi[i] = pseudo_potential(L->rho[i]);
+  }
   if (d->force_kind == LBF_SOURCE_FIELD) {
     if (!d->force_field) {
       lbo_destroy(L);
@@ -684,22 +736,32 @@ int lbo_create(const lbf_case_desc* d, lbo_case** out) {
 /* ---------------------------------------------------- momentum phases */
 
 /* Node::compute_density / compute_velocity(false), src/momentum/node.rs:418-457 */
-static void node_moments(lbo_case* L, long n) {
-  int q = L->q, D = L->d;
+static void node_density(lbo_case* L, long n) {
+  int q = L->q;
   const double* f = L->f + n * q;
   double rho = f[0];
   for (int i = 1; i < q; i++) rho = rho + f[i];
   L->rho[n] = rho;
+  if (L->multiphase) L->phi[n] = pseudo_potential(rho); /* compute_phi, multiphase.rs:48-54 */
+}
+
+static void node_velocity(lbo_case* L, long n) {
+  int q = L->q, D = L->d;
+  const double* f = L->f + n * q;
+  const double rho = L->rho[n];
   for (int x = 0; x < D; x++) {
     double acc = f[0] * (double)L->v.c[0][x];
     for (int i = 1; i < q; i++) acc = acc + f[i] * (double)L->v.c[i][x];
     L->u[n * D + x] = acc / rho;
   }
-  if (L->has_force) {
-    double F[3];
-    get_force(L, n, F);
+  double F[3];
+  if (get_force(L, n, F))
     for (int x = 0; x < D; x++) L->u[n * D + x] += 0.5 * DELTA_T * F[x] / rho;
-  }
+}
+
+static void node_moments(lbo_case* L, long n) {
+  node_density(L, n);
+  node_velocity(L, n);
 }
 
 /* BGK (+ Guo source) node.rs:311-333; TRT lib.rs:118-127; MRT lib.rs:129-139 */
@@ -711,9 +773,8 @@ static void node_collide(lbo_case* L, long n) {
   double* fs = L->fs + n * q;
   if (col->op == LBF_BGK) {
     k_bgk(&L->v, f, feq, col->tau, fs);
-    if (L->has_force) {
-      double F[3], S[27];
-      get_force(L, n, F);
+    double F[3], S[27];
+    if (get_force(L, n, F)) {
       k_momentum_source(&L->v, L->u + n * L->d, F, col->tau, S);
       for (int i = 0; i < q; i++) fs[i] += S[i];
     }
@@ -796,8 +857,12 @@ static int momentum_faces(lbo_case* L) {
 static int momentum_main_steps(lbo_case* L) {
   int q = L->q;
   long nf = L->nfluid;
+  /* update_density_and_velocity_step (lattice.rs:330-335); densities and
+   * psi of every node first (see shan_chen_force) */
 #pragma omp parallel for schedule(static)
-  for (long j = 0; j < nf; j++) node_moments(L, L->fluid[j]);
+  for (long j = 0; j < nf; j++) node_density(L, L->fluid[j]);
+#pragma omp parallel for schedule(static)
+  for (long j = 0; j < nf; j++) node_velocity(L, L->fluid[j]);
 #pragma omp parallel for schedule(static)
   for (long j = 0; j < nf; j++) {
     long n = L->fluid[j];
@@ -1377,6 +1442,7 @@ size_t lbo_sizeof(int which) {
     case 6: return offsetof(lbf_case_desc, shift_scale);
     case 7: return offsetof(lbf_scalar_desc, initial_q_ads);
     case 8: return offsetof(lbf_case_desc, force_field);
+    case 9: return offsetof(lbf_case_desc, wall_density);
   }
   return 0;
 }

@@ -37,6 +37,7 @@ def test_struct_layout_matches_header():
     assert lib.lbo_sizeof(6) == abi.CaseDesc.shift_scale.offset
     assert lib.lbo_sizeof(7) == abi.ScalarDesc.initial_q_ads.offset
     assert lib.lbo_sizeof(8) == abi.CaseDesc.force_field.offset
+    assert lib.lbo_sizeof(9) == abi.CaseDesc.wall_density.offset
 
 
 def test_entry_point_list_equals_header():

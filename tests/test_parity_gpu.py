@@ -44,6 +44,17 @@ def cases():
                                      ScalarFaceBC(abi.SBC_PERIODIC), ScalarFaceBC(abi.SBC_PERIODIC)],
                               inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.2,
                               adsorption=dict(adsorption_rate=0.2, q_max=1.0))]
+    rng = np.random.default_rng(8)
+    sc_mask = np.zeros((20, 16), np.uint8)
+    sc_mask[6:10, 5:9] = 1
+    shan_chen_2d = Spec(velocity_set=abi.D2Q9, n=(20, 16), collision=Collision.bgk(1.0),
+                        faces=[FaceBC.periodic()] * 4, node_type=sc_mask.reshape(-1),
+                        initial_density=0.693 + 0.02 * rng.standard_normal(320),
+                        initial_velocity=(0.0, 0.0), force=(1e-5, 0.0), multiphase=(-5.0, 0.5))
+    shan_chen_3d = Spec(velocity_set=abi.D3Q19, n=(10, 8, 6), collision=Collision.trt(1.1, 0.9),
+                        faces=[FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 4,
+                        initial_density=0.69 + 0.02 * rng.standard_normal(480),
+                        initial_velocity=(0.0, 0.0, 0.0), multiphase=(-4.5, 1.2))
     anti = configs.cavity(n=12, tau=0.9)
     anti.faces = [FaceBC.periodic(), FaceBC.periodic(), FaceBC.anti_bounce_back(1.01),
                   FaceBC.no_slip()]
@@ -60,6 +71,8 @@ def cases():
         "coupled_discs_unknown_only": (configs.coupled_discs(nx=40, ny=20, count=5, rmin=1.5, rmax=3, seed=2, precision=abi.F64, k_a=0.05, zeroth_unknown_only=True), 30),
         "two_scalars_mixed_bcs": (two, 25),
         "anti_bounce_back_walls": (anti, 20),
+        "shan_chen_d2q9_bgk_force_walls": (shan_chen_2d, 40),
+        "shan_chen_d3q19_trt_no_slip": (shan_chen_3d, 16),
     }
 
 
