@@ -385,11 +385,16 @@ __global__ void __launch_bounds__(kBlock)
   const unsigned long long step = ctrl->step;
   double acc[5] = {0, 0, 0, 0, 0};
   // one node per thread (the host launches ceil(N / kBlock) workgroups).
-  // xcd_slab: workgroups are dealt round-robin to the 8 XCDs, so workgroup b
-  // takes node block (b % 8) * (grid / 8) + b / 8 and each XCD sweeps one
-  // contiguous slab; y neighbours then share that XCD's L2 (host: grid % 8 == 0).
+  // xcd_group G: workgroups are dealt round-robin to the 8 XCDs, so XCD k
+  // runs workgroups k, k+8, ...; with G > 0 its t-th workgroup takes node
+  // block ((t / G) * 8 + k) * G + t % G, i.e. each XCD sweeps runs of G
+  // consecutive blocks (neighbouring rows share its L2) while the 8 XCDs
+  // together still sweep one contiguous window (host: grid % (8 G) == 0).
   uint32_t blk = blockIdx.x;
-  if (a.xcd_slab) blk = (blk & 7u) * (gridDim.x >> 3) + (blk >> 3);
+  if (a.xcd_group) {
+    const uint32_t G = (uint32_t)a.xcd_group, t = blk >> 3;
+    blk = ((t / G) * 8u + (blk & 7u)) * G + t % G;
+  }
   const uint32_t n = blk * kBlock + threadIdx.x;
   aa_node<L, St, OP, ODD, MOM>(a, n, step, a.fresh != 0, acc);
   if (a.flags & kFlagResiduals) {
