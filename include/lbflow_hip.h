@@ -113,6 +113,11 @@ extern "C" {
 
 /* ---- case flags ---------------------------------------------------------- */
 #define LBF_CASE_FREEZE_MOMENTUM 1u /* CLI --freeze-momentum (src/cli.rs:40-42) */
+/* Zou-He faces on D3Q15/D3Q19/D3Q27 with this engine's own closure (no
+ * counterpart: the reference panics, src/velocity_set.rs:140-160). Velocity
+ * faces give every velocity component; pressure faces give the density and
+ * the tangential components. */
+#define LBF_CASE_ZOU_HE_3D 2u
 
 /* ---- step flags ---------------------------------------------------------- */
 #define LBF_STEP_RESIDUALS 1u /* residual norms after every step (lattice.rs:369-394) */

@@ -48,6 +48,7 @@ STREAM_AUTO, STREAM_TWO_ARRAY, STREAM_AA = 0, 1, 2
 SOURCE_NONE, SOURCE_UNIFORM, SOURCE_FIELD = 0, 1, 2
 
 CASE_FREEZE_MOMENTUM = 1
+CASE_ZOU_HE_3D = 2
 STEP_RESIDUALS = 1
 STEP_STOP_ON_CONVERGENCE = 2
 

@@ -140,6 +140,7 @@ class Spec:
     streaming: int = abi.STREAM_AUTO
     shift_scale: float = 0.0
     freeze_momentum: bool = False
+    zou_he_3d: bool = False                  # own Zou-He closure on the 3-D lattices
     multiphase: object = None                # None or (G, wall_density): Shan-Chen force
 
     @property
@@ -170,7 +171,8 @@ class Spec:
         dims = list(self.n) + [1] * (3 - len(self.n))
         for i in range(3):
             d.n[i] = int(dims[i]) if i < self.d else 1
-        d.flags = abi.CASE_FREEZE_MOMENTUM if self.freeze_momentum else 0
+        d.flags = (abi.CASE_FREEZE_MOMENTUM if self.freeze_momentum else 0) | \
+            (abi.CASE_ZOU_HE_3D if self.zou_he_3d else 0)
         d.collision = self.collision.to_c()
         N = self.num_nodes
         if self.node_type is not None:

@@ -68,15 +68,23 @@ def coupled_discs(nx=1024, ny=512, count=200, rmin=4.0, rmax=12.0, seed=4, tau=0
 
 
 def sphere_channel(nx=384, ny=128, nz=128, radius=16, u_in=0.05, tau=0.6, precision=abi.F64,
-                   velocity_set=abi.D3Q27, op=None) -> Spec:
-    """Flow past a bounce-back sphere: periodic along x, no-slip walls on y/z,
-    uniform initial velocity u_in along x (no inlet/outlet: the reference has
-    no 3-D Zou-He closure, so the flow decays from its initial state)."""
+                   velocity_set=abi.D3Q27, op=None, inlet_outlet=True) -> Spec:
+    """Flow past a bounce-back sphere of the given radius centred at
+    (nx/4, ny/2, nz/2). inlet_outlet: Zou-He velocity inlet u = (u_in, 0, 0)
+    on the west face and pressure outlet rho = 1 (zero tangential velocity) on
+    the east face, periodic lateral faces; this uses the engine's own 3-D
+    Zou-He closure, which the reference does not have. Otherwise periodic
+    along x and no-slip walls on y/z, starting from a uniform u_in."""
     mask = sphere_mask(nx, ny, nz, (nx // 4, ny // 2, nz // 2), radius)
+    if inlet_outlet:
+        faces = [FaceBC.zou_he(None, (u_in, 0.0, 0.0)), FaceBC.zou_he(1.0, (None, 0.0, 0.0)),
+                 FaceBC.periodic(), FaceBC.periodic(), FaceBC.periodic(), FaceBC.periodic()]
+    else:
+        faces = [FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 4
     return Spec(
         velocity_set=velocity_set, n=(nx, ny, nz), collision=op or Collision.bgk(tau),
-        faces=[FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 4, node_type=mask,
-        initial_density=1.0, initial_velocity=(u_in, 0.0, 0.0), precision=precision)
+        faces=faces, node_type=mask, initial_density=1.0, initial_velocity=(u_in, 0.0, 0.0),
+        precision=precision, zou_he_3d=inlet_outlet)
 
 
 def serving_case(velocity_set=abi.D2Q9, side=64, tau=0.8, porosity=0.8, seed=0,

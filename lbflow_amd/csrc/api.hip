@@ -86,13 +86,16 @@ bool vs_info(int id, VsInfo* v) {
 }
 
 // The AA scheme redirects links across non-periodic faces to the node's own
-// slots, which reproduces no-slip and moving-wall bounce-back exactly; it
-// needs opposite faces to be both periodic or both not, and no scalars.
+// slots, which reproduces no-slip and moving-wall bounce-back exactly, and
+// Zou-He replaces those populations from the node's other ones; it needs
+// opposite faces to be both periodic or both not, and no scalars.
 bool aa_compatible(const lbf_case_desc* d, const VsInfo& v) {
   if (d->n_scalars || d->multiphase) return false;
   for (int f = 0; f < 2 * v.d; f++) {
     const int k = d->faces[f].kind;
-    if (k != LBF_BC_NO_SLIP && k != LBF_BC_BOUNCE_BACK && k != LBF_BC_PERIODIC) return false;
+    if (k != LBF_BC_NO_SLIP && k != LBF_BC_BOUNCE_BACK && k != LBF_BC_PERIODIC &&
+        k != LBF_BC_ZOU_HE)
+      return false;
   }
   for (int a = 0; a < v.d; a++)
     if ((d->faces[2 * a].kind == LBF_BC_PERIODIC) != (d->faces[2 * a + 1].kind == LBF_BC_PERIODIC))
@@ -274,6 +277,18 @@ bool zou_he_ok(int face, unsigned known) {
   return false;
 }
 
+// this engine's own 3-D closure: all velocity components, or the density and
+// the tangential components
+bool zou_he_3d_ok(int face, unsigned known, int D) {
+  const unsigned bits[3] = {LBF_KNOWN_UX, LBF_KNOWN_UY, LBF_KNOWN_UZ};
+  const int a = face / 2;
+  bool tang = true;
+  for (int t = 0; t < D; t++)
+    if (t != a && !(known & bits[t])) tang = false;
+  if (!(known & LBF_KNOWN_DENSITY)) return tang && (known & bits[a]);
+  return tang && !(known & bits[a]);
+}
+
 void fill_collision(KArgs& a, const lbf_collision& col, int q) {
   a.op = col.op;
   a.omega = 1.0 / col.tau;        // DELTA_T / tau, src/kernel.rs:35
@@ -339,10 +354,15 @@ int validate(const lbf_case_desc* d, VsInfo* v) {
     if (k < LBF_BC_NO_SLIP || k > LBF_BC_PERIODIC)
       return fail(LBF_EINVAL, "face %d has no boundary condition", f);
     if (k == LBF_BC_ZOU_HE) {
-      if (d->velocity_set != LBF_D2Q9)
-        return fail(LBF_EUNSUPPORTED, "Zou & He BC computation function not defined.");
-      if (!zou_he_ok(f, d->faces[f].known))
+      if (d->velocity_set != LBF_D2Q9) {
+        if (!(d->flags & LBF_CASE_ZOU_HE_3D))
+          return fail(LBF_EUNSUPPORTED, "Zou & He BC computation function not defined.");
+        if (!zou_he_3d_ok(f, d->faces[f].known, v->d))
+          return fail(LBF_EINVAL, "Zou-He face %d: give u (velocity face) or rho and the "
+                                  "tangential u (pressure face)", f);
+      } else if (!zou_he_ok(f, d->faces[f].known)) {
         return fail(LBF_EINVAL, "Error: Wrong input of Zou & He (1997) boundary conditions.");
+      }
     }
   }
   if (d->precision < LBF_F64 || d->precision > LBF_BF16_SHIFTED)

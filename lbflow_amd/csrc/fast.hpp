@@ -222,7 +222,7 @@ __device__ __forceinline__ void aa_gather(__amdgpu_buffer_rsrc_t rs, uint32_t pl
 }
 
 // face conditions from the redirected values (f_ibar = f*_i on crossing links)
-template <class L, class T>
+template <class L, class T, bool SHIFTED>
 __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
   using k = K<L>;
   using Dv = Derived<L>;
@@ -232,6 +232,20 @@ __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
     constexpr int Fc = decltype(FI)::value;
     if (!(on >> Fc & 1u)) return;
     const FaceK& bc = a.face[Fc];
+    if (bc.kind == 4) {
+      // Zou-He from the node's post-stream populations (the unknown ones hold
+      // the redirected bounce-back values and are replaced); delta-form
+      // storage converts to f around the closure
+      if constexpr (SHIFTED) sfor<L::Q>([&](auto J) { f[decltype(J)::value] += T(k::w(decltype(J)::value)); });
+      if constexpr (L::has_zou_he) {
+        (void)zou_he_d2q9<T>(Fc, f, bc.known, T(bc.density), T(bc.vel[0]), T(bc.vel[1]));
+      } else {
+        const T v3[3] = {T(bc.vel[0]), T(bc.vel[1]), T(bc.vel[2])};
+        zou_he_general<L, T, Fc>(f, bc.known, T(bc.density), v3);
+      }
+      if constexpr (SHIFTED) sfor<L::Q>([&](auto J) { f[decltype(J)::value] -= T(k::w(decltype(J)::value)); });
+      return;
+    }
     T vel[3] = {T(0), T(0), T(0)};
     if (bc.kind == 2)
       sfor<L::D>([&](auto A) { vel[decltype(A)::value] = T(bc.vel[decltype(A)::value]); });
@@ -284,7 +298,7 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
     aa_gather<L, St, ODD>(rs, pb, n, fresh, cls & kLinkMask, fm, p, f);
     if constexpr (St::shifted)
       sfor<Q>([&](auto J) { f[decltype(J)::value] *= isc; });
-    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T>(a, p.on, f);
+    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T, St::shifted>(a, p.on, f);
     // ---- moments
     // (this tier is not held to the reference's summation order)
     T rho, u[3], drho = f[0];
@@ -430,7 +444,7 @@ __global__ void __launch_bounds__(kBlock) momentum_readback_aa(KArgs a) {
     aa_gather<L, St, ODD>(rs, a.plane_bytes, n, fresh, cls & kLinkMask, face_mask<L>(p.on), p, f);
     if constexpr (St::shifted)
       sfor<Q>([&](auto J) { f[decltype(J)::value] *= T(a.inv_scale); });
-    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T>(a, p.on, f);
+    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T, St::shifted>(a, p.on, f);
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;
       outd[(size_t)j * a.ld + n] = St::shifted ? (double)f[j] + k::w(j) : (double)f[j];

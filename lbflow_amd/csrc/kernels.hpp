@@ -217,9 +217,12 @@ __device__ __forceinline__ void apply_momentum_faces(const KArgs& a, uint32_t on
         }
       });
     } else if (bc.kind == 4 /*ZouHe*/) {
+      // combinations the reference panics on are rejected by lbf_create
       if constexpr (L::has_zou_he) {
-        // combinations the reference panics on are rejected by lbf_create
         (void)zou_he_d2q9<T>(F, f, bc.known, T(bc.density), T(bc.vel[0]), T(bc.vel[1]));
+      } else {
+        const T vel[3] = {T(bc.vel[0]), T(bc.vel[1]), T(bc.vel[2])};
+        zou_he_general<L, T, F>(f, bc.known, T(bc.density), vel);
       }
     }
   });

@@ -364,6 +364,81 @@ __device__ __forceinline__ bool zou_he_d2q9(int face, T* f, unsigned known, T de
   return false;
 }
 
+// Zou-He closure for any lattice and face (this engine's own; the reference
+// has one only for D2Q9, src/velocity_set/d2q9.rs:165-245, and panics for
+// the 3-D sets). Face Fc with inward normal sign s along axis a:
+//  - S0 = sum of f_i with c_ia = 0, So = sum of f_i with s c_ia < 0;
+//  - velocity given: rho = (S0 + 2 So) / (1 - s u_a); density given:
+//    s u_a = 1 - (S0 + 2 So) / rho (tangential velocity given);
+//  - unknowns (s c_ia > 0): f_i = f_ibar + 6 w_i rho (c_i . u), which sets
+//    rho and the normal momentum (sum of w_i over the unknowns is 1/6);
+//  - tangential axis t: lambda_t = (rho u_t - sum_i c_it f_i) / sum_U c_it^2
+//    and f_i += c_it lambda_t over the unknowns, which sets the tangential
+//    momentum and leaves rho and the other components unchanged (the unknown
+//    set is symmetric in every tangential axis).
+// On D2Q9 this reproduces the reference's closed forms.
+template <class L, int Fc>
+struct ZouHeFace {
+  static constexpr int a = Fc / 2;
+  static constexpr int s = (Fc % 2 == 0) ? 1 : -1;
+  static constexpr int cn(int i) { return L::c[i][a] * s; }
+  static constexpr int den(int t) {
+    int n = 0;
+    for (int i = 0; i < L::Q; i++)
+      if (cn(i) > 0) n += L::c[i][t] * L::c[i][t];
+    return n;
+  }
+};
+
+template <class L, class T, int Fc>
+__device__ __forceinline__ void zou_he_general(T* f, unsigned known, T density, const T* vel) {
+  using k = K<L>;
+  using Z = ZouHeFace<L, Fc>;
+  constexpr int a = Z::a, D = L::D;
+  T s0 = T(0), so = T(0);
+  sfor<L::Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Z::cn(i) == 0) s0 = s0 + f[i];
+    if constexpr (Z::cn(i) < 0) so = so + f[i];
+  });
+  T u[3] = {vel[0], vel[1], vel[2]};
+  T rho;
+  if (known & 1u) {
+    rho = density;
+    const T un = T(1) - (s0 + T(2) * so) / rho;
+    u[a] = Z::s > 0 ? un : -un;
+  } else {
+    const T un = Z::s > 0 ? u[a] : -u[a];
+    rho = (s0 + T(2) * so) / (T(1) - un);
+  }
+  sfor<L::Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Z::cn(i) > 0) f[i] = f[k::qbar(i)] + T(6) * T(k::w(i)) * rho * udotc<L, i>(u);
+  });
+  T lam[3] = {T(0), T(0), T(0)};
+  sfor<D>([&](auto TI) {
+    constexpr int t = decltype(TI)::value;
+    if constexpr (t != a) {
+      T m = T(0);
+      sfor<L::Q>([&](auto I) {
+        constexpr int i = decltype(I)::value;
+        if constexpr (L::c[i][t] > 0) m = m + f[i];
+        if constexpr (L::c[i][t] < 0) m = m - f[i];
+      });
+      lam[t] = (rho * u[t] - m) / T(Z::den(t));
+    }
+  });
+  sfor<L::Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (Z::cn(i) > 0) {
+      sfor<D>([&](auto TI) {
+        constexpr int t = decltype(TI)::value;
+        if constexpr (t != a && L::c[i][t] != 0) f[i] = f[i] + T(L::c[i][t]) * lam[t];
+      });
+    }
+  });
+}
+
 // ------------------------------------------------------------- shifted form
 // delta-form equilibrium: delta_eq_i = w_i (drho + rho (3 u.c + 4.5 (u.c)^2 - 1.5 u.u))
 template <class L>

@@ -368,6 +368,69 @@ static void k_momentum_source(const vset* v, const double* u, const double* F, d
   }
 }
 
+/* Zou-He closure for the 3-D lattices: this build's own (the reference has
+ * none). The device restates it in node_math.hpp (zou_he_general) with the
+ * same operation order: S0 / So sums, rho or the normal velocity, unknowns
+ * f_i = f_ibar + 6 w_i rho (c_i . u), then the tangential corrections
+ * lambda_t = (rho u_t - sum_i c_it f_i) / sum_U c_it^2 added as c_it lambda_t. */
+static int zou_he_3d_ok(int face, unsigned known, int D) {
+  const unsigned bits[3] = {LBF_KNOWN_UX, LBF_KNOWN_UY, LBF_KNOWN_UZ};
+  int a = face / 2, tang = 1;
+  for (int t = 0; t < D; t++)
+    if (t != a && !(known & bits[t])) tang = 0;
+  if (!(known & LBF_KNOWN_DENSITY)) return tang && (known & bits[a]);
+  return tang && !(known & bits[a]);
+}
+
+static void k_zou_he_general(const vset* v, int face, double* f, unsigned known, double density,
+                             const double* vel) {
+  const int a = face / 2, s = (face % 2 == 0) ? 1 : -1, q = v->q, D = v->d;
+  double s0 = 0.0, so = 0.0;
+  for (int i = 0; i < q; i++) {
+    int cn = v->c[i][a] * s;
+    if (cn == 0) s0 = s0 + f[i];
+    if (cn < 0) so = so + f[i];
+  }
+  double u[3] = {vel[0], vel[1], vel[2]}, rho;
+  if (known & LBF_KNOWN_DENSITY) {
+    rho = density;
+    double un = 1.0 - (s0 + 2.0 * so) / rho;
+    u[a] = s > 0 ? un : -un;
+  } else {
+    double un = s > 0 ? u[a] : -u[a];
+    rho = (s0 + 2.0 * so) / (1.0 - un);
+  }
+  for (int i = 0; i < q; i++) {
+    if (v->c[i][a] * s <= 0) continue;
+    double udc = 0.0;
+    int first = 1;
+    for (int x = 0; x < D; x++) {
+      if (v->c[i][x] == 0) continue;
+      double t = v->c[i][x] > 0 ? u[x] : -u[x];
+      udc = first ? t : udc + t;
+      first = 0;
+    }
+    f[i] = f[v->qbar[i]] + 6.0 * v->w[i] * rho * udc;
+  }
+  double lam[3] = {0, 0, 0};
+  for (int t = 0; t < D; t++) {
+    if (t == a) continue;
+    double m = 0.0;
+    int den = 0;
+    for (int i = 0; i < q; i++) {
+      if (v->c[i][t] > 0) m = m + f[i];
+      if (v->c[i][t] < 0) m = m - f[i];
+      if (v->c[i][a] * s > 0) den += v->c[i][t] * v->c[i][t];
+    }
+    lam[t] = (rho * u[t] - m) / (double)den;
+  }
+  for (int i = 0; i < q; i++) {
+    if (v->c[i][a] * s <= 0) continue;
+    for (int t = 0; t < D; t++)
+      if (t != a && v->c[i][t] != 0) f[i] = f[i] + (double)v->c[i][t] * lam[t];
+  }
+}
+
 /* src/kernel.rs:156-169 */
 static void k_scalar_source(const vset* v, double source, double tau_g, double* S) {
   double coeff_b = 1.0 - 0.5 / tau_g;
@@ -593,8 +656,15 @@ int lbo_create(const lbf_case_desc* d, lbo_case** out) {
       return fail(LBF_EINVAL, "face %d has no boundary condition (Boundary faces not found)", f);
     }
     if (k == LBF_BC_ZOU_HE && L->v.id != LBF_D2Q9) {
-      lbo_destroy(L);
-      return fail(LBF_EUNSUPPORTED, "Zou & He BC computation function not defined.");
+      if (!(d->flags & LBF_CASE_ZOU_HE_3D)) {
+        lbo_destroy(L);
+        return fail(LBF_EUNSUPPORTED, "Zou & He BC computation function not defined.");
+      }
+      if (!zou_he_3d_ok(f, d->faces[f].known, D)) {
+        lbo_destroy(L);
+        return fail(LBF_EINVAL, "Zou-He face %d: give u (velocity face) or rho and the "
+                                "tangential u (pressure face)", f);
+      }
     }
   }
   L->type = (unsigned char*)calloc(N, 1);
@@ -842,7 +912,10 @@ static int momentum_faces(lbo_case* L) {
           }
         } break;
         case LBF_BC_ZOU_HE: /* bc.rs:131-149 */
-          bad |= k_zou_he_d2q9(fc, f, bc->known, bc->density, bc->velocity) != 0;
+          if (v->id == LBF_D2Q9)
+            bad |= k_zou_he_d2q9(fc, f, bc->known, bc->density, bc->velocity) != 0;
+          else
+            k_zou_he_general(v, fc, f, bc->known, bc->density, bc->velocity);
           break;
         default: /* Periodic: nothing (bc.rs:50) */
           break;
@@ -1371,6 +1444,14 @@ int lbo_k_scalar_source(int id, double source, double tau_g, double* out) {
   k_scalar_source(&v, source, tau_g, out);
   return 0;
 }
+int lbo_k_zou_he_general(int vs, int face, double* f, unsigned known, double density,
+                         const double* vel) {
+  vset v;
+  if (vset_init(vs, &v)) return -1;
+  k_zou_he_general(&v, face, f, known, density, vel);
+  return 0;
+}
+
 int lbo_k_zou_he(int face, double* f, unsigned known, double density, const double* vel) {
   return k_zou_he_d2q9(face, f, known, density, vel);
 }

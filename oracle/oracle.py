@@ -39,6 +39,7 @@ EXTRA = [
     ("lbo_k_momentum_source", C.c_int, [C.c_int, _dp, _dp, C.c_double, _dp]),
     ("lbo_k_scalar_source", C.c_int, [C.c_int, C.c_double, C.c_double, _dp]),
     ("lbo_k_zou_he", C.c_int, [C.c_int, _dp, C.c_uint, C.c_double, _dp]),
+    ("lbo_k_zou_he_general", C.c_int, [C.c_int, C.c_int, _dp, C.c_uint, C.c_double, _dp]),
     ("lbo_k_velocity_explicit", C.c_int, [C.c_int, C.c_double, _dp, _dp]),
     ("lbo_neighbor", C.c_long, [C.c_void_p, C.c_long, C.c_int]),
     ("lbo_node_set", C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_long, _dp]),
@@ -163,6 +164,13 @@ def k_zou_he(face, f, known, density=0.0, vel=(0.0, 0.0, 0.0)):
     fa = np.array(f, dtype=np.float64).copy()
     va, vp = arr(list(vel) + [0.0] * (3 - len(vel)))
     rc = load().lbo_k_zou_he(face, fa.ctypes.data_as(_dp), known, density, vp)
+    return rc, fa
+
+
+def k_zou_he_general(vs, face, f, known, density=0.0, vel=(0.0, 0.0, 0.0)):
+    fa = np.array(f, dtype=np.float64).copy()
+    va, vp = arr(list(vel) + [0.0] * (3 - len(vel)))
+    rc = load().lbo_k_zou_he_general(vs, face, fa.ctypes.data_as(_dp), known, density, vp)
     return rc, fa
 
 

@@ -55,6 +55,17 @@ def cases():
                         faces=[FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 4,
                         initial_density=0.69 + 0.02 * rng.standard_normal(480),
                         initial_velocity=(0.0, 0.0, 0.0), multiphase=(-4.5, 1.2))
+    zh_channel = Spec(velocity_set=abi.D3Q19, n=(20, 10, 8), collision=Collision.trt(1.2, 0.9),
+                      faces=[FaceBC.zou_he(None, (0.03, 0.0, 0.0)),
+                             FaceBC.zou_he(1.0, (None, 0.0, 0.0))] + [FaceBC.no_slip()] * 4,
+                      initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0), zou_he_3d=True)
+    zh_mixed = Spec(velocity_set=abi.D3Q15, n=(9, 12, 10), collision=Collision.bgk(0.75),
+                    faces=[FaceBC.periodic(), FaceBC.periodic(),
+                           FaceBC.zou_he(1.01, (0.0, None, 0.02)),
+                           FaceBC.zou_he(None, (0.01, -0.02, 0.0)),
+                           FaceBC.bounce_back(1.0, (0.02, 0.0, 0.0)), FaceBC.no_slip()],
+                    node_type=geometry.bernoulli_mask((9, 12, 10), 0.15, 3),
+                    initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0), zou_he_3d=True)
     anti = configs.cavity(n=12, tau=0.9)
     anti.faces = [FaceBC.periodic(), FaceBC.periodic(), FaceBC.anti_bounce_back(1.01),
                   FaceBC.no_slip()]
@@ -73,6 +84,10 @@ def cases():
         "anti_bounce_back_walls": (anti, 20),
         "shan_chen_d2q9_bgk_force_walls": (shan_chen_2d, 40),
         "shan_chen_d3q19_trt_no_slip": (shan_chen_3d, 16),
+        "zou_he_3d_d3q19_channel": (zh_channel, 30),
+        "zou_he_3d_d3q15_mixed_faces": (zh_mixed, 20),
+        "zou_he_3d_d3q27_sphere": (configs.sphere_channel(nx=24, ny=12, nz=12, radius=3,
+                                                          velocity_set=abi.D3Q27), 20),
     }
 
 

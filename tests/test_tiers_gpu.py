@@ -36,6 +36,9 @@ AA_CASES = {
         [1.0, 1.19, 1.4, 1.0, 1.2, 1.0, 1.2, 1.0, 1.2, 1.25, 1.4, 1.25, 1.4, 1.25, 1.25, 1.25, 1.98, 1.98, 1.98])),
     "d3q27_bgk": configs.porous_channel(n=(10, 8, 9), velocity_set=abi.D3Q27, op=Collision.bgk(0.8), force=1e-4),
     "d3q15_trt": configs.porous_channel(n=(10, 9, 8), velocity_set=abi.D3Q15, force=1e-4),
+    "zou_he_3d_sphere_d3q19": configs.sphere_channel(nx=24, ny=12, nz=10, radius=3,
+                                                     velocity_set=abi.D3Q19, op=Collision.trt(1.1, 0.9)),
+    "zou_he_3d_sphere_d3q27": configs.sphere_channel(nx=20, ny=10, nz=12, radius=3),
 }
 
 
