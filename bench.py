@@ -258,7 +258,7 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
     }
 
 
-def precision_leg(scale=1.0):
+def precision_leg(scale=1.0, only=None):
     """Precision tier per configuration: relative L2 velocity error and global
     mass drift of each storage against the FP64 parity path after the
     configuration's full step count (x scale), plus wall time.
@@ -266,7 +266,8 @@ def precision_leg(scale=1.0):
       cavity   D2Q9 BGK lid-driven cavity 128^2, tau 0.6, lid 0.1, 20,000 steps
       porous   D3Q19 TRT porous channel 256^3, 5,000 steps
       sphere   D3Q27 MRT (this build's basis) flow past a sphere, 384x128x128,
-               periodic x, walls y/z, initial u = 0.05, 10,000 steps
+               radius 16, 3-D Zou-He velocity inlet u = 0.05 and pressure
+               outlet rho = 1, periodic y/z, 10,000 steps
       discs    D2Q9 BGK + D2Q9 scalar with adsorption, 1024x512, 200 discs,
                Zou-He inlet/outlet, 50,000 steps (FP32 two-array vs FP64)
     """
@@ -308,11 +309,13 @@ def precision_leg(scale=1.0):
         ("porous", lambda p: configs.porous_channel(n=256, precision=p), 5000,
          [abi.F32, abi.F32_SHIFTED, abi.F16_SHIFTED, abi.BF16_SHIFTED]),
         ("sphere", lambda p: configs.sphere_channel(precision=p, op=mrt27), 10000,
-         [abi.F32, abi.F16_SHIFTED, abi.BF16_SHIFTED]),
+         [abi.F32, abi.F32_SHIFTED, abi.F16_SHIFTED, abi.BF16_SHIFTED]),
         ("discs", lambda p: configs.coupled_discs(precision=p), 50000, [abi.F32]),
     ]
     report = {}
     for name, make, steps, precs in cases:
+        if only and name not in only:
+            continue
         steps = max(1, int(steps * scale))
         print(f"[precision] {name}: f64 reference, {steps} steps", file=sys.stderr, flush=True)
         ref_spec = make(abi.F64)
@@ -350,6 +353,8 @@ def main():
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-tier report (errors vs FP64 per configuration)")
     ap.add_argument("--precision-scale", type=float, default=1.0)
+    ap.add_argument("--precision-cases", default="",
+                    help="comma-separated subset of cavity,porous,sphere,discs")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -357,7 +362,8 @@ def main():
         sys.exit(2)
 
     if args.precision:
-        print(json.dumps({"precision": precision_leg(args.precision_scale)}))
+        print(json.dumps({"precision": precision_leg(args.precision_scale,
+                                                  [c for c in args.precision_cases.split(",") if c])}))
         return
     from lbflow_amd import abi, configs
     hip = Hip()

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -395,12 +396,24 @@ extern "C" {
 
 const char* lbf_last_error(void) { return g_err.c_str(); }
 
+}  // extern "C"
+
+// Plane stride (elements) of every per-node array: N rounded up to a whole
+// wavefront, plus an optional pad set by lbfx_set_plane_pad for cases created
+// afterwards (0 by default).
+static std::atomic<long long> g_plane_pad{0};
+static long long plane_stride(long long n_nodes) {
+  return (n_nodes + 63) / 64 * 64 + g_plane_pad.load(std::memory_order_relaxed);
+}
+
+extern "C" {
+
 int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
   VsInfo v;
   int rc = validate(d, &v);
   if (rc) return rc;
   const long long N = (long long)d->n[0] * d->n[1] * (v.d == 3 ? d->n[2] : 1);
-  const long long ld = (N + 63) / 64 * 64;
+  const long long ld = plane_stride(N);
   const size_t sb = storage_bytes(d->precision), tb = d->precision == LBF_F64 ? 8 : 4;
   const bool aa_ok = aa_compatible(d, v);
   const bool aa = d->precision >= LBF_F32_SHIFTED || d->streaming == LBF_STREAM_AA ||
@@ -429,7 +442,7 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   c->d = v.d, c->q = v.q;
   c->nx = (int)d->n[0], c->ny = (int)d->n[1], c->nz = v.d == 3 ? (int)d->n[2] : 1;
   c->N = (long long)c->nx * c->ny * c->nz;
-  c->ld = (c->N + 63) / 64 * 64;
+  c->ld = plane_stride(c->N);
   c->prec = d->precision;
   c->sb = storage_bytes(c->prec);
   c->tb = c->prec == LBF_F64 ? 8 : 4;
@@ -440,21 +453,20 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     const bool aa_ok = aa_compatible(d, v);
     if (d->streaming == LBF_STREAM_AA && !aa_ok) {
       delete c;
-      return fail(LBF_EUNSUPPORTED, "AA streaming needs no scalars, face conditions in "
-                                    "{no-slip, bounce-back, periodic} and matching periodicity "
-                                    "of opposite faces");
+      return fail(LBF_EUNSUPPORTED, "AA streaming needs no scalars and no Shan-Chen force, "
+                                    "face conditions in {no-slip, bounce-back, periodic, "
+                                    "Zou-He} and matching periodicity of opposite faces");
     }
     if (shifted && !aa_ok) {
       delete c;
       return fail(LBF_EUNSUPPORTED, "shifted storage runs on the AA kernel only (no scalars, "
-                                    "faces in {no-slip, bounce-back, periodic})");
+                                    "no Shan-Chen force, faces in {no-slip, bounce-back, "
+                                    "periodic, Zou-He})");
     }
     if (d->streaming == LBF_STREAM_AA) c->aa = true;
     else if (d->streaming == LBF_STREAM_AUTO) c->aa = aa_ok && c->prec != LBF_F64;
     if (shifted) c->aa = true;
-    const long long n_nodes = (long long)c->nx * c->ny * c->nz;
-    const long long ld_nodes = (n_nodes + 63) / 64 * 64;
-    if (c->aa && (unsigned long long)ld_nodes * c->q * c->sb >= (1ull << 32)) {
+    if (c->aa && (unsigned long long)c->ld * c->q * c->sb >= (1ull << 32)) {
       if (shifted || d->streaming == LBF_STREAM_AA) {
         delete c;
         return fail(LBF_EUNSUPPORTED, "AA population array must stay below 4 GiB");
@@ -1042,6 +1054,16 @@ int lbf_get_link_tables(lbf_case* c, uint8_t* node_type, uint32_t* bb_link_mask,
 extern "C" int lbfx_set_block_group(lbf_case* c, int g) {
   if (!c || g < 0 || (g > 0 && c->grid % (8 * g) != 0)) return -1;
   c->xcd_group = g;
+  return 0;
+}
+
+// Tuning hook (not part of the reference-equivalent ABI): pad, in elements (a
+// multiple of 64, at most 2^20), added to the plane stride of cases created
+// after the call. Shifts the 19 (or 27) population planes against each other
+// in the HBM address map. Returns -1 for an invalid pad.
+extern "C" int lbfx_set_plane_pad(long long elements) {
+  if (elements < 0 || elements % 64 != 0 || elements > (1ll << 20)) return -1;
+  g_plane_pad.store(elements, std::memory_order_relaxed);
   return 0;
 }
 

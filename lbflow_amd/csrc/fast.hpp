@@ -44,8 +44,10 @@ namespace lbf {
 // meq_k(rho,u) - (M w)_k = drho g_k(0) + rho h_k(u) + (g_k(0) - (M w)_k)
 // with meq_k(rho, u) = rho g_k(u) the reference's closed forms and
 // h_k(u) = g_k(u) - g_k(0) written out so no constant cancels in f32.
+// Shifted storage units (scale * delta): G = scale * drho, B = scale * rho,
+// and deq holds scale * delta_eq.
 template <class L>
-__device__ __forceinline__ void meq_shifted(float drho, float rho, const float* u,
+__device__ __forceinline__ void meq_shifted(float G, float B, float sc, const float* u,
                                             const float* deq, float* m) {
   using Dv = Derived<L>;
   auto mw = [](int k) {
@@ -61,7 +63,7 @@ __device__ __forceinline__ void meq_shifted(float drho, float rho, const float* 
     sfor<9>([&](auto K) {
       constexpr int k = decltype(K)::value;
       constexpr float c = float(g0[k] - mw(k));
-      m[k] = drho * float(g0[k]) + rho * h[k] + c;
+      m[k] = G * float(g0[k]) + B * h[k] + sc * c;
     });
   } else if constexpr (L::id == 3) {
     const float ux = u[0], uy = u[1], uz = u[2], u2 = ux * ux + uy * uy + uz * uz;
@@ -74,7 +76,7 @@ __device__ __forceinline__ void meq_shifted(float drho, float rho, const float* 
     sfor<19>([&](auto K) {
       constexpr int k = decltype(K)::value;
       constexpr float c = float(g0[k] - mw(k));
-      m[k] = drho * float(g0[k]) + rho * h[k] + c;
+      m[k] = G * float(g0[k]) + B * h[k] + sc * c;
     });
   } else if constexpr (L::id == 2) {
     const float ux = u[0], uy = u[1], uz = u[2], u2 = ux * ux + uy * uy + uz * uz;
@@ -85,7 +87,7 @@ __device__ __forceinline__ void meq_shifted(float drho, float rho, const float* 
     sfor<15>([&](auto K) {
       constexpr int k = decltype(K)::value;
       constexpr float c = float(g0[k] - mw(k));
-      m[k] = drho * float(g0[k]) + rho * h[k] + c;
+      m[k] = G * float(g0[k]) + B * h[k] + sc * c;
     });
   } else {
     // no closed form: m_eq = M f_eq, so m_eq - M w = M delta_eq
@@ -133,6 +135,22 @@ constexpr uint32_t face_dirs(int F) {
   for (int i = 0; i < L::Q; i++)
     if (Derived<L>::in_face(F, i)) m |= 1u << i;
   return m;
+}
+
+// i leads the opposite pair (i, ibar): the rest direction and the second
+// member of each pair are not leads
+template <class L>
+constexpr bool pair_lead(int i) {
+  return i != 0 && i < K<L>::qbar(i);
+}
+
+// 1 / x: the hardware reciprocal (1 ulp) in f32, a division in f64
+template <class T>
+__device__ __forceinline__ T recip(T x) {
+  if constexpr (sizeof(T) == 4)
+    return __builtin_amdgcn_rcpf(x);
+  else
+    return T(1) / x;
 }
 
 struct NodePos {
@@ -185,8 +203,9 @@ __device__ __forceinline__ uint32_t face_mask(uint32_t on) {
   return m;
 }
 
-// Post-stream populations of node n (compute type, delta form when shifted).
-// links: bit i = neighbour x + c_i (periodic wrap) is solid; fm: face_mask.
+// Post-stream populations of node n in the compute type (shifted storage:
+// scale * (f - w), the units the step runs in). links: bit i = neighbour
+// x + c_i (periodic wrap) is solid; fm: face_mask.
 template <class L, class St, bool ODD>
 __device__ __forceinline__ void aa_gather(__amdgpu_buffer_rsrc_t rs, uint32_t plane_bytes,
                                           uint32_t n, bool fresh, uint32_t links, uint32_t fm,
@@ -217,13 +236,14 @@ __device__ __forceinline__ void aa_gather(__amdgpu_buffer_rsrc_t rs, uint32_t pl
       const uint32_t mine = own + j * plane_bytes;
       v = bload<S>(rs, opaque((sel >> jb & 1u) ? mine : nb), 0u);
     }
-    f[j] = T(v);  // shifted storage: scaled by the caller
+    f[j] = T(v);
   });
 }
 
 // face conditions from the redirected values (f_ibar = f*_i on crossing links)
+// (shifted storage: f holds scale * (f - w); sc / isc convert)
 template <class L, class T, bool SHIFTED>
-__device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
+__device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T sc, T isc, T* f) {
   using k = K<L>;
   using Dv = Derived<L>;
   T fsv[L::Q];
@@ -236,14 +256,22 @@ __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
       // Zou-He from the node's post-stream populations (the unknown ones hold
       // the redirected bounce-back values and are replaced); delta-form
       // storage converts to f around the closure
-      if constexpr (SHIFTED) sfor<L::Q>([&](auto J) { f[decltype(J)::value] += T(k::w(decltype(J)::value)); });
+      if constexpr (SHIFTED)
+        sfor<L::Q>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          f[j] = f[j] * isc + T(k::w(j));
+        });
       if constexpr (L::has_zou_he) {
         (void)zou_he_d2q9<T>(Fc, f, bc.known, T(bc.density), T(bc.vel[0]), T(bc.vel[1]));
       } else {
         const T v3[3] = {T(bc.vel[0]), T(bc.vel[1]), T(bc.vel[2])};
         zou_he_general<L, T, Fc>(f, bc.known, T(bc.density), v3);
       }
-      if constexpr (SHIFTED) sfor<L::Q>([&](auto J) { f[decltype(J)::value] -= T(k::w(decltype(J)::value)); });
+      if constexpr (SHIFTED)
+        sfor<L::Q>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          f[j] = (f[j] - T(k::w(j))) * sc;
+        });
       return;
     }
     T vel[3] = {T(0), T(0), T(0)};
@@ -256,7 +284,8 @@ __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T* f) {
         f[ib] = fsv[ib];
       } else if (bc.kind == 2) {
         const T udc = udotc<L, i>(vel);
-        f[ib] = fsv[ib] - T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+        const T dv = T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+        f[ib] = fsv[ib] - (SHIFTED ? sc * dv : dv);
       }
     });
   });
@@ -274,6 +303,7 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
   using S = typename St::S;
   using k = K<L>;
   constexpr int Q = L::Q, D = L::D;
+  static_assert(k::qbar(0) == 0 && !k::any_c(0), "direction 0 is the rest direction");
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(a.pop[0], (short)0, (int)0xffffffff, 0x00020000);
   const uint32_t pb = a.plane_bytes;
@@ -296,31 +326,43 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
     const uint32_t fm = face_mask<L>(p.on);
     T f[Q];
     aa_gather<L, St, ODD>(rs, pb, n, fresh, cls & kLinkMask, fm, p, f);
-    if constexpr (St::shifted)
-      sfor<Q>([&](auto J) { f[decltype(J)::value] *= isc; });
-    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T, St::shifted>(a, p.on, f);
-    // ---- moments
+    // shifted storage: f holds scale * (f - w) and the step runs in those units
+    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T, St::shifted>(a, p.on, sc, isc, f);
+    // ---- moments over opposite pairs (i, ibar): sp = f_i + f_ibar, dm = f_i - f_ibar
     // (this tier is not held to the reference's summation order)
-    T rho, u[3], drho = f[0];
-    sfor<Q - 1>([&](auto J) { drho += f[decltype(J)::value + 1]; });
-    rho = St::shifted ? T(1) + drho : drho;
+    T sp[Q], dm[Q];
+    T sum = f[0];
+    sfor<Q>([&](auto J) {
+      constexpr int i = decltype(J)::value;
+      if constexpr (pair_lead<L>(i)) {
+        sp[i] = f[i] + f[k::qbar(i)];
+        dm[i] = f[i] - f[k::qbar(i)];
+        sum += sp[i];
+      }
+    });
+    const T rho = St::shifted ? T(1) + sum * isc : sum;
+    const T irho = recip(rho);
+    const T ju = St::shifted ? isc * irho : irho;
+    T u[3];
     sfor<D>([&](auto A) {
       constexpr int ax = decltype(A)::value;
       T s = T(0);
       sfor<Q>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        constexpr int c = k::ca(j, ax);
-        if constexpr (c > 0) s += f[j];
-        if constexpr (c < 0) s -= f[j];
+        constexpr int i = decltype(J)::value;
+        if constexpr (pair_lead<L>(i)) {
+          constexpr int c = k::ca(i, ax);
+          if constexpr (c > 0) s += dm[i];
+          if constexpr (c < 0) s -= dm[i];
+        }
       });
-      u[ax] = s / rho;
+      u[ax] = s * ju;
     });
     T Fv[3] = {T(0), T(0), T(0)};
     if (a.force_kind) {
       sfor<D>([&](auto A) {
         constexpr int ax = decltype(A)::value;
         Fv[ax] = a.force_kind == 1 ? T(a.F[ax]) : T(a.src_field[(size_t)ax * ld + n]);
-        u[ax] += T(0.5) * Fv[ax] / rho;
+        u[ax] += T(0.5) * Fv[ax] * irho;
       });
     }
     if (store_m) {
@@ -339,31 +381,61 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
         acc[4] += bad ? 1.0 : 0.0;
       }
     }
-    // ---- collision
-    T feq[Q], fs[Q];
-    if constexpr (St::shifted)
-      equilibrium_shifted<L>(drho, rho, u, feq);
-    else
-      equilibrium<L>(rho, u, feq);
-    if constexpr (OP == 0) {
-      bgk<L>(f, feq, omega, omega_p, fs);
-      if (a.force_kind) add_guo<L>(u, Fv, coeff_b, fs);
-    } else if constexpr (OP == 1) {
-      trt<L>(f, feq, wp, wm, fs);
+    // ---- collision. f_eq,i = w_i (A + B t_i), t_i = 3 u.c_i + 4.5 (u.c_i)^2 - 1.5 u.u,
+    // with A = B = rho for plain storage and A = scale * drho, B = scale * rho in
+    // shifted units; over a pair, f_eq,i +- f_eq,ibar = 2 sym_i, 2 anti_i with
+    // sym_i = w_i (base + 4.5 B (u.c_i)^2), base = A - 1.5 B u.u, anti_i = 3 w_i B u.c_i
+    const T B = St::shifted ? sc * rho : rho;
+    const T base = (St::shifted ? sum : rho) - T(1.5) * B * udotu<L>(u);
+    T fs[Q];
+    if constexpr (OP == 1) {
+      // TRT (src/kernel.rs:44-65): f*_i = f_i - w+ (f+ - f_eq+) - w- (f- - f_eq-)
+      const T hp = T(0.5) * wp, hm = T(0.5) * wm;
+      fs[0] = f[0] - wp * (f[0] - T(k::w(0)) * base);
+      sfor<Q>([&](auto J) {
+        constexpr int i = decltype(J)::value;
+        if constexpr (pair_lead<L>(i)) {
+          constexpr int ib = k::qbar(i);
+          const T udc = udotc<L, i>(u);
+          const T sym2 = T(2.0 * k::w(i)) * base + (T(9.0 * k::w(i)) * B) * udc * udc;
+          const T anti2 = (T(6.0 * k::w(i)) * B) * udc;
+          const T S = hp * (sp[i] - sym2);
+          const T Dd = hm * (dm[i] - anti2);
+          fs[i] = f[i] - S - Dd;
+          fs[ib] = f[ib] - S + Dd;
+        }
+      });
     } else {
-      T s[Q];
-      sfor<Q>([&](auto KI) { s[decltype(KI)::value] = T(a.s[decltype(KI)::value]); });
-      if constexpr (St::shifted) {
-        T m[Q], meq[Q], ms[Q];
-        sfor<Q>([&](auto Kr) { m[decltype(Kr)::value] = mrt_dot<L, decltype(Kr)::value>(f); });
-        meq_shifted<L>(drho, rho, u, feq, meq);
-        sfor<Q>([&](auto Kr) {
-          constexpr int kk = decltype(Kr)::value;
-          ms[kk] = m[kk] - s[kk] * (m[kk] - meq[kk]);
-        });
-        sfor<Q>([&](auto I) { fs[decltype(I)::value] = mrt_inv_dot<L, decltype(I)::value>(ms); });
+      T feq[Q];
+      feq[0] = T(k::w(0)) * base;
+      sfor<Q>([&](auto J) {
+        constexpr int i = decltype(J)::value;
+        if constexpr (pair_lead<L>(i)) {
+          const T udc = udotc<L, i>(u);
+          const T sym = T(k::w(i)) * base + (T(4.5 * k::w(i)) * B) * udc * udc;
+          const T anti = (T(3.0 * k::w(i)) * B) * udc;
+          feq[i] = sym + anti;
+          feq[k::qbar(i)] = sym - anti;
+        }
+      });
+      if constexpr (OP == 0) {
+        bgk<L>(f, feq, omega, omega_p, fs);
+        if (a.force_kind) add_guo<L>(u, Fv, St::shifted ? coeff_b * sc : coeff_b, fs);
       } else {
-        mrt<L>(rho, u, f, feq, s, fs);
+        T s[Q];
+        sfor<Q>([&](auto KI) { s[decltype(KI)::value] = T(a.s[decltype(KI)::value]); });
+        if constexpr (St::shifted) {
+          T m[Q], meq[Q], ms[Q];
+          sfor<Q>([&](auto Kr) { m[decltype(Kr)::value] = mrt_dot<L, decltype(Kr)::value>(f); });
+          meq_shifted<L>(sum, B, sc, u, feq, meq);
+          sfor<Q>([&](auto Kr) {
+            constexpr int kk = decltype(Kr)::value;
+            ms[kk] = m[kk] - s[kk] * (m[kk] - meq[kk]);
+          });
+          sfor<Q>([&](auto I) { fs[decltype(I)::value] = mrt_inv_dot<L, decltype(I)::value>(ms); });
+        } else {
+          mrt<L>(rho, u, f, feq, s, fs);
+        }
       }
     }
     // ---- scatter
@@ -371,9 +443,7 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;
       constexpr int jb = k::qbar(j);
-      T v = fs[j];
-      if constexpr (St::shifted) v = v * sc;
-      const S sv = static_cast<S>(v);
+      const S sv = static_cast<S>(fs[j]);
       if (!ODD) {
         bstore<S>(sv, rs, own, jb * pb);
       } else {
@@ -392,7 +462,9 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
 // compiled for full occupancy.
 template <class L, class St, int OP, bool ODD, bool MOM>
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(MOM ? 2 : (sizeof(typename St::T) == 8 ? 4 : (ODD ? 6 : 8)))))
+    __attribute__((amdgpu_waves_per_eu(MOM ? 2
+                                       : (sizeof(typename St::T) == 8 || L::Q > 19) ? 4
+                                       : (ODD ? 6 : 8))))
     momentum_step_aa(KArgs a) {
   Ctrl* ctrl = a.ctrl;
   if (ctrl->stopped) return;
@@ -442,12 +514,12 @@ __global__ void __launch_bounds__(kBlock) momentum_readback_aa(KArgs a) {
     node_pos<L>(a, n, true, p);
     T f[Q];
     aa_gather<L, St, ODD>(rs, a.plane_bytes, n, fresh, cls & kLinkMask, face_mask<L>(p.on), p, f);
-    if constexpr (St::shifted)
-      sfor<Q>([&](auto J) { f[decltype(J)::value] *= T(a.inv_scale); });
-    if (p.on && !fresh && !a.faces_noslip_only) aa_faces<L, T, St::shifted>(a, p.on, f);
+    if (p.on && !fresh && !a.faces_noslip_only)
+      aa_faces<L, T, St::shifted>(a, p.on, T(a.scale), T(a.inv_scale), f);
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;
-      outd[(size_t)j * a.ld + n] = St::shifted ? (double)f[j] + k::w(j) : (double)f[j];
+      outd[(size_t)j * a.ld + n] =
+          St::shifted ? (double)f[j] * a.inv_scale + k::w(j) : (double)f[j];
     });
   }
 }

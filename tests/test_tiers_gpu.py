@@ -115,3 +115,4 @@ def test_aa_set_pops_odd_and_even():
     g.main_steps(5)
     o.main_steps(5)
     assert rel(g.pops(), o.pops()) < 1e-12
+
