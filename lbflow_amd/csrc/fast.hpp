@@ -144,6 +144,100 @@ constexpr bool pair_lead(int i) {
   return i != 0 && i < K<L>::qbar(i);
 }
 
+// u . F over the lattice's dimensions
+template <class L, class T>
+__device__ __forceinline__ T udotu_F(const T* u, const T* F) {
+  T s = u[0] * F[0] + u[1] * F[1];
+  if constexpr (L::D == 3) s = s + u[2] * F[2];
+  return s;
+}
+
+// Parity of row k of the MRT matrix under c -> -c: +1 if M(k, ibar) =
+// M(k, i) for all i, -1 if M(k, ibar) = -M(k, i), 0 otherwise. Rows of
+// fixed parity act on pair sums (even) or pair differences (odd) only.
+template <class L>
+constexpr int mrt_row_parity(int kr) {
+  bool even = true, odd = true;
+  for (int i = 0; i < L::Q; i++) {
+    const double a = Derived<L>::M(kr, i), b = Derived<L>::M(kr, K<L>::qbar(i));
+    if (a != b) even = false;
+    if (a != -b) odd = false;
+  }
+  return even ? 1 : (odd ? -1 : 0);
+}
+template <class L>
+constexpr bool mrt_rows_have_parity() {
+  for (int kr = 0; kr < L::Q; kr++)
+    if (mrt_row_parity<L>(kr) == 0) return false;
+  return true;
+}
+
+// (M f)_k from the rest value f0 and the pair sums sp / differences dm of
+// the pair leads (half the products of a dense row)
+template <class L, int Krow, class T>
+__device__ __forceinline__ T mrt_dot_pairs(T f0, const T* sp, const T* dm) {
+  constexpr int par = mrt_row_parity<L>(Krow);
+  static_assert(par != 0, "MRT row without parity");
+  T s = T(0);
+  bool first = true;
+  auto add = [&](T t) {
+    if (first) {
+      s = t;
+      first = false;
+    } else {
+      s = s + t;
+    }
+  };
+  if constexpr (par > 0 && Derived<L>::M(Krow, 0) != 0.0) add(T(Derived<L>::M(Krow, 0)) * f0);
+  sfor<L::Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (pair_lead<L>(i)) {
+      constexpr double m = Derived<L>::M(Krow, i);
+      if constexpr (m != 0.0) add(T(m) * (par > 0 ? sp[i] : dm[i]));
+    }
+  });
+  return s;
+}
+
+// f = M^-1 ms over pairs: f_i, f_ibar = S_i +- A_i with S_i from the even
+// rows and A_i from the odd rows (M^-1(ibar, k) = parity_k M^-1(i, k))
+template <class L, class T>
+__device__ __forceinline__ void mrt_inv_pairs(const T* ms, T* fs) {
+  using Dv = Derived<L>;
+  auto part = [&](auto I, auto W) -> T {
+    constexpr int i = decltype(I)::value;
+    T s = T(0);
+    bool first = true;
+    sfor<L::Q>([&](auto Kc) {
+      constexpr int kk = decltype(Kc)::value;
+      constexpr double m = Dv::Minv(i, kk);
+      if constexpr (m != 0.0) {
+        if constexpr (mrt_row_parity<L>(kk) == decltype(W)::value) {
+          const T t = T(m) * ms[kk];
+          if (first) {
+            s = t;
+            first = false;
+          } else {
+            s = s + t;
+          }
+        }
+      }
+    });
+    return s;
+  };
+  using even = std::integral_constant<int, 1>;
+  using odd = std::integral_constant<int, -1>;
+  fs[0] = part(std::integral_constant<int, 0>{}, even{});
+  sfor<L::Q>([&](auto I) {
+    constexpr int i = decltype(I)::value;
+    if constexpr (pair_lead<L>(i)) {
+      const T S = part(I, even{}), A = part(I, odd{});
+      fs[i] = S + A;
+      fs[K<L>::qbar(i)] = S - A;
+    }
+  });
+}
+
 // 1 / x: the hardware reciprocal (1 ulp) in f32, a division in f64
 template <class T>
 __device__ __forceinline__ T recip(T x) {
@@ -405,37 +499,75 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
           fs[ib] = f[ib] - S + Dd;
         }
       });
-    } else {
-      T feq[Q];
-      feq[0] = T(k::w(0)) * base;
+    } else if constexpr (OP == 0) {
+      // BGK (src/kernel.rs:28-42) with the Guo source (src/kernel.rs:122-154)
+      // S_i = b w_i (3 (c_i - u) + 9 (u.c_i) c_i).F, over a pair the parts
+      // b w_i (9 (u.c_i)(c_i.F) - 3 u.F) (even) and 3 b w_i c_i.F (odd)
+      const bool force = a.force_kind != 0;
+      const T bs = St::shifted ? coeff_b * sc : coeff_b;
+      const T uF = udotu_F<L>(u, Fv);
+      fs[0] = omega_p * f[0] + omega * (T(k::w(0)) * base);
+      if (force) fs[0] += bs * T(k::w(0)) * (T(-3.0) * uF);
       sfor<Q>([&](auto J) {
         constexpr int i = decltype(J)::value;
         if constexpr (pair_lead<L>(i)) {
+          constexpr int ib = k::qbar(i);
           const T udc = udotc<L, i>(u);
           const T sym = T(k::w(i)) * base + (T(4.5 * k::w(i)) * B) * udc * udc;
           const T anti = (T(3.0 * k::w(i)) * B) * udc;
-          feq[i] = sym + anti;
-          feq[k::qbar(i)] = sym - anti;
+          T gi = omega_p * f[i] + omega * (sym + anti);
+          T gb = omega_p * f[ib] + omega * (sym - anti);
+          if (force) {
+            const T cF = udotc<L, i>(Fv);
+            const T se = (bs * T(k::w(i))) * (T(9.0) * udc * cF - T(3.0) * uF);
+            const T so = (bs * T(3.0 * k::w(i))) * cF;
+            gi += se + so;
+            gb += se - so;
+          }
+          fs[i] = gi;
+          fs[ib] = gb;
         }
       });
-      if constexpr (OP == 0) {
-        bgk<L>(f, feq, omega, omega_p, fs);
-        if (a.force_kind) add_guo<L>(u, Fv, St::shifted ? coeff_b * sc : coeff_b, fs);
+    } else {
+      // MRT (src/kernel.rs:67-120): f* = M^-1 (m - s (m - m_eq)), every
+      // product over pair sums / differences (each row of M has a parity)
+      static_assert(mrt_rows_have_parity<L>(), "pair-split MRT needs rows of fixed parity");
+      T s[Q], ms[Q];
+      sfor<Q>([&](auto KI) { s[decltype(KI)::value] = T(a.s[decltype(KI)::value]); });
+      if constexpr (L::id == 1 || L::id == 2 || L::id == 3) {
+        // m_eq from the reference's closed forms (shifted: scaled, minus M w)
+        T m[Q], meq[Q];
+        sfor<Q>([&](auto Kr) {
+          m[decltype(Kr)::value] = mrt_dot_pairs<L, decltype(Kr)::value>(f[0], sp, dm);
+        });
+        if constexpr (St::shifted)
+          meq_shifted<L>(sum, B, sc, u, nullptr, meq);
+        else
+          meq_closed<L, T>(rho, u, nullptr, meq);
+        sfor<Q>([&](auto Kr) {
+          constexpr int kk = decltype(Kr)::value;
+          ms[kk] = m[kk] - s[kk] * (m[kk] - meq[kk]);
+        });
+        mrt_inv_pairs<L>(ms, fs);
       } else {
-        T s[Q];
-        sfor<Q>([&](auto KI) { s[decltype(KI)::value] = T(a.s[decltype(KI)::value]); });
-        if constexpr (St::shifted) {
-          T m[Q], meq[Q], ms[Q];
-          sfor<Q>([&](auto Kr) { m[decltype(Kr)::value] = mrt_dot<L, decltype(Kr)::value>(f); });
-          meq_shifted<L>(sum, B, sc, u, feq, meq);
-          sfor<Q>([&](auto Kr) {
-            constexpr int kk = decltype(Kr)::value;
-            ms[kk] = m[kk] - s[kk] * (m[kk] - meq[kk]);
-          });
-          sfor<Q>([&](auto I) { fs[decltype(I)::value] = mrt_inv_dot<L, decltype(I)::value>(ms); });
-        } else {
-          mrt<L>(rho, u, f, feq, s, fs);
-        }
+        // no closed form, m_eq = M f_eq: m - s (m - m_eq) = M f - s M (f - f_eq),
+        // so f* = f - M^-1 (s M (f - f_eq)), two products on f - f_eq
+        T nsp[Q], ndm[Q], dfs[Q];
+        const T n0 = f[0] - T(k::w(0)) * base;
+        sfor<Q>([&](auto J) {
+          constexpr int i = decltype(J)::value;
+          if constexpr (pair_lead<L>(i)) {
+            const T udc = udotc<L, i>(u);
+            nsp[i] = sp[i] - (T(2.0 * k::w(i)) * base + (T(9.0 * k::w(i)) * B) * udc * udc);
+            ndm[i] = dm[i] - (T(6.0 * k::w(i)) * B) * udc;
+          }
+        });
+        sfor<Q>([&](auto Kr) {
+          constexpr int kk = decltype(Kr)::value;
+          ms[kk] = s[kk] * mrt_dot_pairs<L, kk>(n0, nsp, ndm);
+        });
+        mrt_inv_pairs<L>(ms, dfs);
+        sfor<Q>([&](auto J) { fs[decltype(J)::value] = f[decltype(J)::value] - dfs[decltype(J)::value]; });
       }
     }
     // ---- scatter
@@ -463,6 +595,7 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
 template <class L, class St, int OP, bool ODD, bool MOM>
 __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(MOM ? 2
+                                       : (sizeof(typename St::T) == 8 && L::Q > 19) ? 2
                                        : (sizeof(typename St::T) == 8 || L::Q > 19) ? 4
                                        : (ODD ? 6 : 8))))
     momentum_step_aa(KArgs a) {
