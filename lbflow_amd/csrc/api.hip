@@ -515,7 +515,9 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   // every workgroup writes its own residual partials
   c->grid = (int)std::max(1LL, blocks);
   // in-place kernel: runs of 4 node blocks per XCD (neighbouring rows then
-  // share an L2); 256^3 D3Q19 FP32 measured 570 -> 535 us per step
+  // share an L2); 256^3 D3Q19 FP32 measured 570 -> 535 us per step. The
+  // two-array kernels keep the round-robin order: there runs of 1-32 blocks
+  // cut HBM reads by up to a fifth yet cost 0-6 % in time (measured).
   c->xcd_group = (c->aa && c->grid % 32 == 0) ? 4 : 0;
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
@@ -701,6 +703,7 @@ KArgs momentum_args(lbf_case* c) {
 KArgs scalar_args(lbf_case* c, int s) {
   KArgs a = c->base;
   ScalarDev& S = c->sc[s];
+  a.xcd_group = 0;
   a.field = 1 + s;
   a.pop[0] = S.pop[0], a.pop[1] = S.pop[1];
   a.c[0] = S.c[0], a.c[1] = S.c[1];

@@ -60,7 +60,7 @@ struct KArgs {
   int fresh;             // AA: the array holds post-stream f in natural slots
   int aa_odd;            // AA: parity of the step being launched
   uint32_t plane_bytes;  // AA: bytes per population plane (array < 4 GiB)
-  int xcd_group;         // AA: runs of blocks per XCD (0 = round-robin order)
+  int xcd_group;         // runs of node blocks per XCD (0 = round-robin order)
   void* pop[2];          // ping-pong population buffers (two-array scheme)
   void* out;             // stream-only destination
   const uint32_t* cls;

@@ -38,6 +38,21 @@
 #include "kernels.hpp"
 #include "node_math.hpp"
 
+// Waves per SIMD the streaming variant of the in-place kernel is compiled
+// for (D3Q19 and smaller lattices), by storage width and step parity.
+#ifndef LBF_WAVES_16_EVEN
+#define LBF_WAVES_16_EVEN 8
+#endif
+#ifndef LBF_WAVES_16_ODD
+#define LBF_WAVES_16_ODD 6
+#endif
+#ifndef LBF_WAVES_32_EVEN
+#define LBF_WAVES_32_EVEN 8
+#endif
+#ifndef LBF_WAVES_32_ODD
+#define LBF_WAVES_32_ODD 6
+#endif
+
 namespace lbf {
 
 // equilibrium moments minus M w, in delta form, for shifted MRT:
@@ -597,24 +612,17 @@ __global__ void __launch_bounds__(kBlock)
     __attribute__((amdgpu_waves_per_eu(MOM ? 2
                                        : (sizeof(typename St::T) == 8 && L::Q > 19) ? 2
                                        : (sizeof(typename St::T) == 8 || L::Q > 19) ? 4
-                                       : (ODD ? 6 : 8))))
+                                       : sizeof(typename St::S) == 2
+                                           ? (ODD ? LBF_WAVES_16_ODD : LBF_WAVES_16_EVEN)
+                                           : (ODD ? LBF_WAVES_32_ODD : LBF_WAVES_32_EVEN))))
     momentum_step_aa(KArgs a) {
   Ctrl* ctrl = a.ctrl;
   if (ctrl->stopped) return;
   const unsigned long long step = ctrl->step;
   double acc[5] = {0, 0, 0, 0, 0};
-  // one node per thread (the host launches ceil(N / kBlock) workgroups).
-  // xcd_group G: workgroups are dealt round-robin to the 8 XCDs, so XCD k
-  // runs workgroups k, k+8, ...; with G > 0 its t-th workgroup takes node
-  // block ((t / G) * 8 + k) * G + t % G, i.e. each XCD sweeps runs of G
-  // consecutive blocks (neighbouring rows share its L2) while the 8 XCDs
-  // together still sweep one contiguous window (host: grid % (8 G) == 0).
-  uint32_t blk = blockIdx.x;
-  if (a.xcd_group) {
-    const uint32_t G = (uint32_t)a.xcd_group, t = blk >> 3;
-    blk = ((t / G) * 8u + (blk & 7u)) * G + t % G;
-  }
-  const uint32_t n = blk * kBlock + threadIdx.x;
+  // one node per thread (the host launches ceil(N / kBlock) workgroups),
+  // node blocks in XCD runs (block_index)
+  const uint32_t n = block_index(a) * kBlock + threadIdx.x;
   aa_node<L, St, OP, ODD, MOM>(a, n, step, a.fresh != 0, acc);
   if (a.flags & kFlagResiduals) {
     __shared__ double lds[kBlock / 64];

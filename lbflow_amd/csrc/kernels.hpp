@@ -228,6 +228,21 @@ __device__ __forceinline__ void apply_momentum_faces(const KArgs& a, uint32_t on
   });
 }
 
+// Node block of this workgroup. Workgroups are dealt round-robin to the 8
+// XCDs (each with its own L2), so XCD k runs workgroups k, k+8, ...; with
+// xcd_group G > 0 its t-th workgroup takes block ((t / G) * 8 + k) * G + t % G:
+// each XCD sweeps runs of G consecutive blocks (neighbouring rows share its
+// L2) while the 8 XCDs together still sweep one contiguous window. A
+// permutation of [0, gridDim) when gridDim % (8 G) == 0 (the host checks).
+__device__ __forceinline__ uint32_t block_index(const KArgs& a) {
+  uint32_t blk = blockIdx.x;
+  if (a.xcd_group) {
+    const uint32_t G = (uint32_t)a.xcd_group, t = blk >> 3;
+    blk = ((t / G) * 8u + (blk & 7u)) * G + t % G;
+  }
+  return blk;
+}
+
 // Plain-form momentum step (storage holds f). Used by the f64 parity path
 // and the plain FP32 path.
 template <class L, class St, int OP>
@@ -255,7 +270,7 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
           coeff_b = T(a.coeff_b);
   double acc[5] = {0, 0, 0, 0, 0};
   const long long ld = a.ld;
-  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < a.N;
+  for (long long n = (long long)block_index(a) * kBlock + threadIdx.x; n < a.N;
        n += (long long)gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
     if (cls & kSolidBit) continue;
@@ -462,7 +477,7 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
   double acc = 0.0;
   const T omega = T(a.omega), omega_p = T(a.omega_p), wp = T(a.wp), wm = T(a.wm),
           coeff_b = T(a.coeff_b);
-  for (long long n = (long long)blockIdx.x * kBlock + threadIdx.x; n < a.N;
+  for (long long n = (long long)block_index(a) * kBlock + threadIdx.x; n < a.N;
        n += (long long)gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
     if (cls & kSolidBit) continue;
