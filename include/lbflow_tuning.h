@@ -1,0 +1,44 @@
+/* lbflow_tuning.h — benchmark and tuning hooks of the HIP engine.
+ *
+ * None of these has a counterpart in the reference (lbflow solves on the CPU
+ * and has no FFI); the Rust binding does not need them. bench.py and the
+ * tuning probes use them to time kernels on the case's own stream, to measure
+ * the device's attainable copy bandwidth and to try launch orders and array
+ * layouts. Same conventions as lbflow_hip.h: 0 = OK, negative = error.
+ */
+#ifndef LBFLOW_TUNING_H
+#define LBFLOW_TUNING_H
+
+#include <stddef.h>
+
+#include "lbflow_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Copies `bytes` (a multiple of 16) from src to dst (device pointers) on the
+ * HIP stream `stream` (NULL = default stream) with a 16-byte-per-lane kernel:
+ * the attainable-bandwidth reference and the calibration kernel for the
+ * rocprofv3 FETCH_SIZE / WRITE_SIZE counters. */
+int lbfx_copy(void* dst, const void* src, size_t bytes, void* stream);
+
+/* The HIP stream (hipStream_t) a case enqueues its launches on, so callers can
+ * bracket its work with events. */
+void* lbfx_case_stream(lbf_case* c);
+
+/* Workgroup order of the in-place kernel: runs of g consecutive node blocks
+ * per XCD (0 = plain round-robin). Returns -1 when the case's grid is not a
+ * multiple of 8 g. */
+int lbfx_set_block_group(lbf_case* c, int g);
+
+/* Pad, in elements (a multiple of 64, at most 2^20), added to the plane
+ * stride of cases created after the call (process-wide; 0 by default).
+ * Returns -1 for an invalid pad. */
+int lbfx_set_plane_pad(long long elements);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBFLOW_TUNING_H */

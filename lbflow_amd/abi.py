@@ -151,6 +151,14 @@ class ServerStats(C.Structure):
                 ("reserved", C.c_uint32)]
 
 
+# benchmark and tuning hooks (include/lbflow_tuning.h; no reference counterpart)
+TUNING_ENTRY_POINTS = [
+    ("lbfx_copy", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("lbfx_case_stream", _vp, [_vp]),
+    ("lbfx_set_block_group", C.c_int, [_vp, C.c_int]),
+    ("lbfx_set_plane_pad", C.c_int, [C.c_longlong]),
+]
+
 SERVER_ENTRY_POINTS = [
     ("lbf_server_create", C.c_int, [C.POINTER(ServerConfig), C.POINTER(_vp)]),
     ("lbf_server_destroy", None, [_vp]),

@@ -42,6 +42,7 @@ def load_library(path: Optional[str] = None):
     lib = C.CDLL(p)
     abi.declare(lib, abi.ENTRY_POINTS)
     abi.declare(lib, abi.SERVER_ENTRY_POINTS)
+    abi.declare(lib, abi.TUNING_ENTRY_POINTS)
     if path is None:
         _lib = lib
     return lib

@@ -21,7 +21,7 @@ def header_functions():
         if fn.endswith(".h"):
             text = open(os.path.join(ROOT, "include", fn)).read()
             text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-            names |= set(re.findall(r"\b(lbf_\w+)\s*\(", text))
+            names |= set(re.findall(r"\b(lbfx?_\w+)\s*\(", text))
     return names
 
 
@@ -41,7 +41,8 @@ def test_struct_layout_matches_header():
 
 
 def test_entry_point_list_equals_header():
-    assert header_functions() == {n for n, _, _ in abi.ENTRY_POINTS + abi.SERVER_ENTRY_POINTS}
+    assert header_functions() == {n for n, _, _ in abi.ENTRY_POINTS + abi.SERVER_ENTRY_POINTS +
+                                  abi.TUNING_ENTRY_POINTS}
 
 
 def test_library_exports_every_declared_function():
