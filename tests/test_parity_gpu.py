@@ -174,3 +174,51 @@ def test_freeze_momentum():
     o.main_steps(10, residuals=True)
     assert rel_err(g.scalar(0)[0], o.scalar(0)[0]) <= RTOL
     assert g.residuals()[0] == 0.0
+
+
+def tiny_cases():
+    rng = np.random.default_rng(11)
+    m3 = np.zeros((3, 2, 2), np.uint8)
+    m3[1, 0, 1] = 1
+    return {
+        # one-node-wide axes wrap onto themselves (rem_euclid neighbours)
+        "d2q9_1x5_periodic": (Spec(velocity_set=abi.D2Q9, n=(1, 5), collision=Collision.bgk(0.7),
+                                   faces=[FaceBC.periodic()] * 4, initial_density=1.0,
+                                   initial_velocity=rng.uniform(-0.02, 0.02, (5, 2))), 9),
+        "d2q9_3x2_walls": (configs.cavity(n=3, tau=0.7).replace(n=(3, 2)), 9),
+        "d3q19_2x2x3_solid": (configs.periodic_box(n=(2, 2, 3)).replace(
+            node_type=m3.reshape(-1), initial_velocity=rng.uniform(-0.02, 0.02, (12, 3))), 9),
+        "d3q27_1x1x4": (configs.periodic_box(n=(1, 1, 4), velocity_set=abi.D3Q27).replace(
+            initial_velocity=rng.uniform(-0.02, 0.02, (4, 3))), 9),
+    }
+
+
+@pytest.mark.parametrize("name", list(tiny_cases()))
+@pytest.mark.parametrize("streaming", [abi.STREAM_TWO_ARRAY, abi.STREAM_AA])
+def test_tiny_lattices(name, streaming):
+    sp, steps = tiny_cases()[name]
+    sp = sp.replace(precision=abi.F64, streaming=streaming)
+    print(f"[tiny] {name} streaming={streaming}", flush=True)
+    g, o = Case(sp), oracle.OracleCase(sp)
+    t1, m1, f1 = g.link_tables()
+    t2, m2, f2 = o.link_tables()
+    assert np.array_equal(t1, t2) and np.array_equal(m1, m2) and np.array_equal(f1, f2)
+    g.initialize_nodes()
+    o.initialize_nodes()
+    g.main_steps(steps)
+    o.main_steps(steps)
+    assert rel_err(g.pops(), o.pops()) <= RTOL
+    rg, ug = g.fields()
+    ro, uo = o.fields()
+    assert rel_err(rg, ro) <= RTOL and rel_err(ug, uo) <= RTOL
+
+
+def test_in_place_array_limit_is_refused_before_allocation():
+    # a D3Q27 FP32-shifted population array of 4 GiB or more cannot use 32-bit
+    # byte offsets: refused with EUNSUPPORTED before any device allocation
+    from lbflow_amd.engine import LbfError
+    sp = configs.periodic_box(n=(400, 400, 250), velocity_set=abi.D3Q27).replace(
+        precision=abi.F32_SHIFTED)
+    with pytest.raises(LbfError) as e:
+        Case(sp)
+    assert e.value.code == abi.EUNSUPPORTED
