@@ -1,16 +1,15 @@
 // serve.hip — serving tier: continuous batching of resident cases on one GPU.
 //
-// Four actors share a server:
+// Three kinds of threads share a server:
 //  - callers submit requests (descriptor copy + step count) into a FIFO;
 //  - the worker thread admits requests while the HBM budget and slots allow
 //    (first in first out, with backfill that never eats into what the oldest
 //    waiting request needs), activates prepared cases into slots, and runs
 //    rounds: one graph replay of K steps of every resident case, then retires
 //    the cases that finished;
-//  - the preparation thread creates and initialises admitted cases
-//    (uploads, node classes, f = f_eq) so the GPU loop never waits on that
-//    work;
-//  - the finishing thread downloads finished cases' fields and frees them.
+//  - four helper threads create and initialise admitted cases (uploads,
+//    node classes, f = f_eq) and download and free finished ones, release
+//    first, so the GPU loop never waits on that work.
 // Case set-up and release issue synchronous copies, so they never overlap a
 // graph capture (hip_mu).
 // Device tables (slots, per-type tile lists) are rewritten only between
@@ -333,8 +332,8 @@ struct lbf_server {
       *out = it->second;
       return 0;
     }
-    // case set-up on the preparation thread issues synchronous copies; keep it
-    // out of the capture window
+    // case set-up and release on the helper threads issue synchronous
+    // copies; keep them out of the capture window
     std::unique_lock<std::shared_mutex> hl(hip_mu);
     hipError_t eb = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
     if (eb != hipSuccess) return err(LBF_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(eb));
@@ -390,7 +389,7 @@ struct lbf_server {
     }
   }
 
-  // Detaches a finished (or failed) case from its slot; the finishing thread
+  // Detaches a finished (or failed) case from its slot; a helper thread
   // downloads its result and frees it, so the next round does not wait.
   void retire(Request* r, int rc) {
     if (r->slot >= 0) {
