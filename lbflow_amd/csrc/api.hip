@@ -963,8 +963,7 @@ int lbf_get_scalar(lbf_case* c, uint32_t s, double* value, double* q_ads) {
 
 int lbf_get_pops(lbf_case* c, int32_t field, double* out) {
   if (!c || !out) return fail(LBF_EINVAL, "NULL argument");
-  if (field < 0 || field > c->ns || (field == 0 && c->frozen && false))
-    return fail(LBF_EINVAL, "no field %d", field);
+  if (field < 0 || field > c->ns) return fail(LBF_EINVAL, "no field %d", field);
   const size_t bytes = (size_t)c->q * c->ld * 8;
   {
     Ctrl h;  // synchronises and resynchronises the host step count
