@@ -49,6 +49,7 @@ class Hip:
         L.hipEventDestroy.argtypes = [C.c_void_p]
         L.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
         L.hipFree.argtypes = [C.c_void_p]
+        L.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.hipDeviceSynchronize.argtypes = []
 
     def event(self):
@@ -86,6 +87,31 @@ def copy_bandwidth(hip, mib=2048, reps=10):
     hip.lib.hipFree(a)
     hip.lib.hipFree(b)
     return 2 * nbytes * reps / (ms / 1e3) / 1e9
+
+
+def sweep_bandwidth(hip, n_nodes, es, q=19, reps=10):
+    """Rate of the in-place kernel's access pattern alone (lbfx_sweep: every
+    lane reads its element of each of q planes and writes it to the paired
+    plane, no arithmetic, no bounce-back) over n_nodes elements of es bytes,
+    GB/s counted as bytes read + written: the attainable bound for that
+    pattern on this device."""
+    from lbflow_amd.engine import load_library
+    lib = load_library()
+    ld = (n_nodes + 63) // 64 * 64
+    nbytes = q * ld * es
+    a = C.c_void_p()
+    assert hip.lib.hipMalloc(C.byref(a), nbytes) == 0
+    assert hip.lib.hipMemset(a, 0, nbytes) == 0
+    for _ in range(2):
+        assert lib.lbfx_sweep(a, q, ld, es, n_nodes, None) == 0
+    e0, e1 = hip.event(), hip.event()
+    hip.record(e0, None)
+    for _ in range(reps):
+        assert lib.lbfx_sweep(a, q, ld, es, n_nodes, None) == 0
+    hip.record(e1, None)
+    ms = hip.elapsed_ms(e0, e1)
+    hip.lib.hipFree(a)
+    return 2 * q * n_nodes * es * reps / (ms / 1e3) / 1e9
 
 
 def time_case(spec, steps, warmup, hip):
@@ -381,6 +407,10 @@ def main():
     results = {}
     copy_gbs = copy_bandwidth(hip)
     print(f"[bench] device copy: {copy_gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    sweep_gbs = {es: sweep_bandwidth(hip, n ** 3, es) for es in (2, 4, 8)}
+    print("[bench] in-place 19-plane sweep: " +
+          ", ".join(f"{8 * es}-bit {g:.0f} GB/s" for es, g in sweep_gbs.items()),
+          file=sys.stderr, flush=True)
     for name, prec, streaming, sbytes in variants:
         sp = configs.porous_channel(n=n, precision=prec, streaming=streaming)
         print(f"[bench] {name}: D3Q19 TRT porous {n}^3, warmup {args.warmup}, steps {args.steps}",
@@ -391,7 +421,8 @@ def main():
         results[name] = dict(mlups=round(mlups, 1), ms_per_step=round(ms / args.steps, 4),
                              achieved_gbs=round(gbs, 1), hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
                              wall_s=round(wall, 3), nonfinite=nonfinite, mass=mass,
-                             bytes_per_site=2 * 19 * sbytes,
+                             bytes_per_site=2 * 19 * sbytes, storage_bytes=sbytes,
+                             frac_of_sweep=round(gbs / sweep_gbs[sbytes], 4),
                              streaming="aa" if streaming == abi.STREAM_AA else "two-array")
         print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)",
               file=sys.stderr, flush=True)
@@ -424,10 +455,13 @@ def main():
                      "traffic": traffic.get(head_name) if n == 256 else None,
                      "algorithmic_bytes_per_launch": bytes_per_launch,
                      "attainable_copy_gbs": round(copy_gbs, 1),
-                     "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4)},
+                     "frac_of_copy": round(head["achieved_gbs"] / copy_gbs, 4),
+                     "attainable_sweep_gbs": round(sweep_gbs[head["storage_bytes"]], 1),
+                     "frac_of_sweep": round(head["achieved_gbs"] / sweep_gbs[head["storage_bytes"]], 4)},
         "cpu_baseline": cpu,
         "serving": serving,
-        "extra": dict(results, copy_gbs=round(copy_gbs, 1)),
+        "extra": dict(results, copy_gbs=round(copy_gbs, 1),
+                      sweep_gbs={f"{8 * es}-bit": round(g, 1) for es, g in sweep_gbs.items()}),
     }
     print(json.dumps(line))
 

@@ -23,6 +23,15 @@ extern "C" {
  * rocprofv3 FETCH_SIZE / WRITE_SIZE counters. */
 int lbfx_copy(void* dst, const void* src, size_t bytes, void* stream);
 
+/* In-place sweep of q (9, 15, 19 or 27) planes of ld elements of es (2, 4
+ * or 8) bytes from `base` (device memory, q * ld * es below 4 GiB), the
+ * first n elements of each, on `stream`: every lane reads its element of each
+ * plane and writes it to the paired plane, with the in-place kernel's buffer
+ * addressing. That kernel's access pattern without arithmetic or
+ * bounce-back, so its rate bounds that kernel's attainable bandwidth.
+ * Returns -1 for an unsupported q, es or size. */
+int lbfx_sweep(void* base, int q, long long ld, int es, long long n, void* stream);
+
 /* The HIP stream (hipStream_t) a case enqueues its launches on, so callers can
  * bracket its work with events. */
 void* lbfx_case_stream(lbf_case* c);

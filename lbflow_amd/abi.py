@@ -154,6 +154,7 @@ class ServerStats(C.Structure):
 # benchmark and tuning hooks (include/lbflow_tuning.h; no reference counterpart)
 TUNING_ENTRY_POINTS = [
     ("lbfx_copy", C.c_int, [_vp, _vp, C.c_size_t, _vp]),
+    ("lbfx_sweep", C.c_int, [_vp, C.c_int, C.c_longlong, C.c_int, C.c_longlong, _vp]),
     ("lbfx_case_stream", _vp, [_vp]),
     ("lbfx_set_block_group", C.c_int, [_vp, C.c_int]),
     ("lbfx_set_plane_pad", C.c_int, [C.c_longlong]),

@@ -200,7 +200,74 @@ hipError_t launch_qads_init(const uint32_t* cls, double* q, long long ld, double
   return hipGetLastError();
 }
 
+// The in-place kernel's access pattern without its arithmetic or bounce-back:
+// every lane reads its element of each of Q planes and writes it to the
+// paired plane (plane j -> plane Q-j, plane 0 onto itself), one element per
+// lane, like the even step, with the same buffer addressing. Its rate bounds
+// what the in-place kernel can reach.
+template <class S, int Q>
+__global__ void __launch_bounds__(kBlock) sweep_kernel(void* base, uint32_t plane_bytes, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const __amdgpu_buffer_rsrc_t r =
+      __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)0xffffffff, 0x00020000);
+  const uint32_t own = i * (uint32_t)sizeof(S);
+  S v[Q];
+  sfor<Q>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if constexpr (sizeof(S) == 8)
+      v[j] = __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b64(r, own, j * plane_bytes, 0));
+    else if constexpr (sizeof(S) == 4)
+      v[j] = __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b32(r, own, j * plane_bytes, 0));
+    else
+      v[j] = __builtin_bit_cast(S, __builtin_amdgcn_raw_buffer_load_b16(r, own, j * plane_bytes, 0));
+  });
+  sfor<Q>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    constexpr int jb = j == 0 ? 0 : Q - j;
+    if constexpr (sizeof(S) == 8) {
+      typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v[j]), r, own, jb * plane_bytes, 0);
+    } else if constexpr (sizeof(S) == 4) {
+      __builtin_amdgcn_raw_buffer_store_b32(v[j], r, own, jb * plane_bytes, 0);
+    } else {
+      __builtin_amdgcn_raw_buffer_store_b16(v[j], r, own, jb * plane_bytes, 0);
+    }
+  });
+}
+
+template <class S>
+hipError_t launch_sweep(void* base, int q, uint32_t plane_bytes, uint32_t n, hipStream_t s) {
+  const dim3 g((n + kBlock - 1) / kBlock), b(kBlock);
+  switch (q) {
+    case 9: hipLaunchKernelGGL((sweep_kernel<S, 9>), g, b, 0, s, base, plane_bytes, n); break;
+    case 15: hipLaunchKernelGGL((sweep_kernel<S, 15>), g, b, 0, s, base, plane_bytes, n); break;
+    case 19: hipLaunchKernelGGL((sweep_kernel<S, 19>), g, b, 0, s, base, plane_bytes, n); break;
+    case 27: hipLaunchKernelGGL((sweep_kernel<S, 27>), g, b, 0, s, base, plane_bytes, n); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 }  // namespace lbf
+
+// Benchmark hook (not part of the reference-equivalent ABI): in-place sweep
+// of q (9, 15, 19 or 27) planes of `ld` elements of es (2, 4 or 8) bytes, the
+// first n elements of each, on `stream` (the array must stay below 4 GiB).
+extern "C" int lbfx_sweep(void* base, int q, long long ld, int es, long long n, void* stream) {
+  if (!base || n < 0 || n > ld || (es != 2 && es != 4 && es != 8)) return -1;
+  if ((unsigned long long)q * ld * es >= (1ull << 32)) return -1;
+  if (n == 0) return 0;
+  const uint32_t pb = (uint32_t)(ld * es);
+  hipError_t e;
+  if (es == 2)
+    e = lbf::launch_sweep<uint16_t>(base, q, pb, (uint32_t)n, (hipStream_t)stream);
+  else if (es == 4)
+    e = lbf::launch_sweep<uint32_t>(base, q, pb, (uint32_t)n, (hipStream_t)stream);
+  else
+    e = lbf::launch_sweep<uint64_t>(base, q, pb, (uint32_t)n, (hipStream_t)stream);
+  return e == hipSuccess ? 0 : (e == hipErrorInvalidValue ? -1 : -3);
+}
 
 // Benchmark hook (not part of the reference-equivalent ABI): copies `bytes`
 // (a multiple of 16) from src to dst on `stream`.
