@@ -253,6 +253,80 @@ __device__ __forceinline__ void mrt_inv_pairs(const T* ms, T* fs) {
   });
 }
 
+// D3Q27's own MRT basis (lattice.hpp) has tensor-product rows
+// p_o1(cx) p_o2(cy) p_o3(cz) with p_0 = 1, p_1(a) = a, p_2(a) = 3a^2 - 2, so
+// M x is three passes of the 3-point transform along x, y and z over the
+// directions arranged as a 3x3x3 cube, and M^-1 likewise with p_o(a) / |p_o|^2
+// (|p_o|^2 = 3, 2, 6): about 250 operations instead of two dense 27x27
+// products. Returns dfs = M^-1 (s . (M neq)).
+template <class L>
+constexpr int cube_dir(int x, int y, int z) {
+  for (int i = 0; i < L::Q; i++)
+    if (L::c[i][0] == x - 1 && L::c[i][1] == y - 1 && L::c[i][2] == z - 1) return i;
+  return -1;
+}
+template <class L>
+constexpr int tensor_row(int ox, int oy, int oz) {
+  for (int k = 0; k < L::Q; k++)
+    if (L::ord[k][0] == ox && L::ord[k][1] == oy && L::ord[k][2] == oz) return k;
+  return -1;
+}
+template <class L, class T>
+__device__ __forceinline__ void mrt_tensor_relax(const T* neq, const T* s, T* dfs) {
+  static_assert(L::Q == 27 && L::p(0, -1) == 1 && L::p(1, -1) == -1 && L::p(2, -1) == 1 &&
+                    L::p(2, 0) == -2 && L::p(1, 1) == 1 && L::p(2, 1) == 1,
+                "tensor-product MRT basis p0 = 1, p1 = a, p2 = 3a^2 - 2");
+  T g[27];  // g[x + 3 (y + 3 z)], x, y, z = c + 1
+  sfor<27>([&](auto I) {
+    constexpr int q = decltype(I)::value;
+    constexpr int i = cube_dir<L>(q % 3, (q / 3) % 3, q / 9);
+    static_assert(i >= 0, "D3Q27 direction");
+    g[q] = neq[i];
+  });
+  // forward along axis A (stride 1, 3, 9): (a0, a1, a2) -> (a0 + a1 + a2,
+  // a2 - a0, a0 + a2 - 2 a1), order index in place of the coordinate
+  auto fwd = [&](auto SI) {
+    constexpr int st = decltype(SI)::value;
+    sfor<9>([&](auto LI) {
+      constexpr int l = decltype(LI)::value;
+      constexpr int b = st == 1 ? 3 * l : (st == 3 ? (l % 3) + 9 * (l / 3) : l);
+      const T a0 = g[b], a1 = g[b + st], a2 = g[b + 2 * st];
+      const T e = a0 + a2;
+      g[b] = e + a1;
+      g[b + st] = a2 - a0;
+      g[b + 2 * st] = e - T(2) * a1;
+    });
+  };
+  auto inv = [&](auto SI) {
+    constexpr int st = decltype(SI)::value;
+    sfor<9>([&](auto LI) {
+      constexpr int l = decltype(LI)::value;
+      constexpr int b = st == 1 ? 3 * l : (st == 3 ? (l % 3) + 9 * (l / 3) : l);
+      const T m0 = g[b] * T(1.0 / 3.0), m1 = g[b + st] * T(0.5), m2 = g[b + 2 * st] * T(1.0 / 6.0);
+      const T e = m0 + m2;
+      g[b] = e - m1;
+      g[b + st] = m0 - T(2) * m2;
+      g[b + 2 * st] = e + m1;
+    });
+  };
+  fwd(std::integral_constant<int, 1>{});
+  fwd(std::integral_constant<int, 3>{});
+  fwd(std::integral_constant<int, 9>{});
+  sfor<27>([&](auto I) {
+    constexpr int q = decltype(I)::value;
+    constexpr int kk = tensor_row<L>(q % 3, (q / 3) % 3, q / 9);
+    static_assert(kk >= 0, "tensor row");
+    g[q] = s[kk] * g[q];
+  });
+  inv(std::integral_constant<int, 9>{});
+  inv(std::integral_constant<int, 3>{});
+  inv(std::integral_constant<int, 1>{});
+  sfor<27>([&](auto I) {
+    constexpr int q = decltype(I)::value;
+    dfs[cube_dir<L>(q % 3, (q / 3) % 3, q / 9)] = g[q];
+  });
+}
+
 // 1 / x: the hardware reciprocal (1 ulp) in f32, a division in f64
 template <class T>
 __device__ __forceinline__ T recip(T x) {
@@ -567,21 +641,39 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
       } else {
         // no closed form, m_eq = M f_eq: m - s (m - m_eq) = M f - s M (f - f_eq),
         // so f* = f - M^-1 (s M (f - f_eq)), two products on f - f_eq
-        T nsp[Q], ndm[Q], dfs[Q];
-        const T n0 = f[0] - T(k::w(0)) * base;
-        sfor<Q>([&](auto J) {
-          constexpr int i = decltype(J)::value;
-          if constexpr (pair_lead<L>(i)) {
-            const T udc = udotc<L, i>(u);
-            nsp[i] = sp[i] - (T(2.0 * k::w(i)) * base + (T(9.0 * k::w(i)) * B) * udc * udc);
-            ndm[i] = dm[i] - (T(6.0 * k::w(i)) * B) * udc;
-          }
-        });
-        sfor<Q>([&](auto Kr) {
-          constexpr int kk = decltype(Kr)::value;
-          ms[kk] = s[kk] * mrt_dot_pairs<L, kk>(n0, nsp, ndm);
-        });
-        mrt_inv_pairs<L>(ms, dfs);
+        T dfs[Q];
+        if constexpr (L::id == 4) {
+          // tensor-product basis: separable transforms on f - f_eq
+          T neq[Q];
+          neq[0] = f[0] - T(k::w(0)) * base;
+          sfor<Q>([&](auto J) {
+            constexpr int i = decltype(J)::value;
+            if constexpr (pair_lead<L>(i)) {
+              const T udc = udotc<L, i>(u);
+              const T sym = T(k::w(i)) * base + (T(4.5 * k::w(i)) * B) * udc * udc;
+              const T anti = (T(3.0 * k::w(i)) * B) * udc;
+              neq[i] = f[i] - (sym + anti);
+              neq[k::qbar(i)] = f[k::qbar(i)] - (sym - anti);
+            }
+          });
+          mrt_tensor_relax<L>(neq, s, dfs);
+        } else {
+          T nsp[Q], ndm[Q];
+          const T n0 = f[0] - T(k::w(0)) * base;
+          sfor<Q>([&](auto J) {
+            constexpr int i = decltype(J)::value;
+            if constexpr (pair_lead<L>(i)) {
+              const T udc = udotc<L, i>(u);
+              nsp[i] = sp[i] - (T(2.0 * k::w(i)) * base + (T(9.0 * k::w(i)) * B) * udc * udc);
+              ndm[i] = dm[i] - (T(6.0 * k::w(i)) * B) * udc;
+            }
+          });
+          sfor<Q>([&](auto Kr) {
+            constexpr int kk = decltype(Kr)::value;
+            ms[kk] = s[kk] * mrt_dot_pairs<L, kk>(n0, nsp, ndm);
+          });
+          mrt_inv_pairs<L>(ms, dfs);
+        }
         sfor<Q>([&](auto J) { fs[decltype(J)::value] = f[decltype(J)::value] - dfs[decltype(J)::value]; });
       }
     }
