@@ -171,7 +171,8 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
     [32, 256] (D2Q9) and [32, side_3d_max] (D3Q19), tau ~ U[0.55, 1.2],
     steps ~ U{1000, 20000} x step_scale, porosity ~ U[0.6, 0.9], FP32 or
     FP16-shifted 50/50, periodic faces, BGK.
-      - throughput: all `count` requests submitted at once, served cases/s;
+      - throughput: all `count` requests submitted at once, each result
+        fetched as its case completes, served cases/s;
         the plain baseline runs a uniform random sample of `baseline_sample`
         cases one after another (create, f64 two-array steps, fields) and
         scales its time to the whole mix;
@@ -222,10 +223,12 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
     t0 = time.perf_counter()
     with Server() as srv:
         tickets = [srv.submit(sp, n) for sp, n in specs]
-        srv.drain(3600)
-        st = srv.stats()
+        # the client takes each result (rho, u) as its case completes, in
+        # submission order, while the server keeps running the others
         for t in tickets:
+            srv.wait(t, 3600)
             srv.result(t)
+        st = srv.stats()
     wall = time.perf_counter() - t0
     served_cps = count / wall
 
@@ -280,7 +283,8 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
         "served_p99_s": round(float(np.percentile(lat, 99)), 4),
         "plain_p50_s": round(float(np.percentile(plain_lat, 50)), 4),
         "plain_p99_s": round(float(np.percentile(plain_lat, 99)), 4),
-        "graphs_built": int(st["graphs_built"] + st_lat["graphs_built"]),
+        "graphs_built_throughput": int(st["graphs_built"]),
+        "graphs_built_latency": int(st_lat["graphs_built"]),
     }
 
 

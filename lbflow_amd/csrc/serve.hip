@@ -7,11 +7,13 @@
 //    waiting request needs), activates prepared cases into slots, and runs
 //    rounds: one graph replay of K steps of every resident case, then retires
 //    the cases that finished;
-//  - four helper threads create and initialise admitted cases (uploads,
+//  - eight helper threads create and initialise admitted cases (uploads,
 //    node classes, f = f_eq) and download and free finished ones, release
 //    first, so the GPU loop never waits on that work.
 // Case set-up and release issue synchronous copies, so they never overlap a
-// graph capture (hip_mu).
+// graph capture (hip_mu). A pending capture closes a gate in front of the
+// helpers' shared sections, so a steady stream of set-ups and releases cannot
+// keep the capture waiting; it waits only for the sections already running.
 // Device tables (slots, per-type tile lists) are rewritten only between
 // rounds, on the worker's stream, so a captured graph of K steps for a given
 // mix of case types stays valid while cases come and go.
@@ -54,7 +56,7 @@ namespace {
 
 // case types: lattice (D2Q9, D3Q19) x storage (F32, F32S, F16S, BF16S) x op (BGK, TRT)
 constexpr int kTypes = 16;
-constexpr int kHelpers = 4;  // threads that set up and release cases
+constexpr int kHelpers = 8;  // threads that set up and release cases
 constexpr uint32_t kGridCandidates[] = {512, 1024, 2048, 4096, 8192};
 constexpr int kNumCandidates = sizeof(kGridCandidates) / sizeof(kGridCandidates[0]);
 
@@ -112,6 +114,9 @@ struct lbf_server {
 
   std::mutex mu;
   std::shared_mutex hip_mu;  // graph capture (exclusive) vs case set-up / release
+  std::mutex gate_mu;        // capture_wanted: no new shared section may start
+  std::condition_variable gate_cv;
+  bool capture_wanted = false;
   std::condition_variable cv_work, cv_help, cv_done;
   std::map<uint64_t, std::unique_ptr<Request>> reqs;
   std::deque<Request*> queue, to_prep, ready;
@@ -202,6 +207,15 @@ struct lbf_server {
     cv_work.notify_one();
   }
 
+  // Shared section of a helper: waits while a capture is pending, then holds
+  // hip_mu shared for the caller's scope.
+  std::shared_lock<std::shared_mutex> enter_shared() {
+    std::unique_lock<std::mutex> g(gate_mu);
+    gate_cv.wait(g, [&] { return !capture_wanted; });
+    g.unlock();
+    return std::shared_lock<std::shared_mutex>(hip_mu);
+  }
+
   // --------------------------------------------------------------- preparation
   void do_prep(Request* r) {
     {
@@ -215,7 +229,7 @@ struct lbf_server {
       int rc;
       const double tp = now();
       {
-        std::shared_lock<std::shared_mutex> hl(hip_mu);
+        auto hl = enter_shared();
         rc = lbf_create(&d, &c);
         if (!rc) rc = lbf_initialize_nodes(c);
         if (!rc) rc = lbf_synchronize(c);
@@ -333,7 +347,22 @@ struct lbf_server {
       return 0;
     }
     // case set-up and release on the helper threads issue synchronous
-    // copies; keep them out of the capture window
+    // copies; keep them out of the capture window. The gate stays closed until
+    // this function returns (after hl is released).
+    {
+      std::lock_guard<std::mutex> g(gate_mu);
+      capture_wanted = true;
+    }
+    struct GateOpen {
+      lbf_server* p;
+      ~GateOpen() {
+        {
+          std::lock_guard<std::mutex> g(p->gate_mu);
+          p->capture_wanted = false;
+        }
+        p->gate_cv.notify_all();
+      }
+    } gate_open{this};
     std::unique_lock<std::shared_mutex> hl(hip_mu);
     hipError_t eb = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
     if (eb != hipSuccess) return err(LBF_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(eb));
@@ -409,7 +438,7 @@ struct lbf_server {
       int rc = job.second;
       const double tr = now();
       {
-        std::shared_lock<std::shared_mutex> hl(hip_mu);
+        auto hl = enter_shared();
         if (!rc) {
           case_note_external_steps(r->c, r->steps);
           r->rho.resize(r->nodes);
@@ -752,22 +781,35 @@ int lbf_server_wait(lbf_server* s, uint64_t ticket, double timeout_s) {
 int lbf_server_result(lbf_server* s, uint64_t ticket, double* rho, double* u, double* pops,
                       int32_t release) {
   if (!s) return err(LBF_EINVAL, "NULL argument");
-  std::lock_guard<std::mutex> lk(s->mu);
-  auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) return err(LBF_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
-  Request& r = *it->second;
-  if (r.state == LBF_REQ_FAILED)
-    return err(r.error ? r.error : LBF_EDEVICE, "request %llu failed: %s",
-               (unsigned long long)ticket, r.errmsg.c_str());
-  if (r.state != LBF_REQ_DONE)
-    return err(LBF_ESTATE, "request %llu is not done", (unsigned long long)ticket);
+  std::unique_ptr<Request> owned;  // released request, copied after the lock
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(ticket);
+    if (it == s->reqs.end()) return err(LBF_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+    Request& r = *it->second;
+    if (r.state == LBF_REQ_FAILED)
+      return err(r.error ? r.error : LBF_EDEVICE, "request %llu failed: %s",
+                 (unsigned long long)ticket, r.errmsg.c_str());
+    if (r.state != LBF_REQ_DONE)
+      return err(LBF_ESTATE, "request %llu is not done", (unsigned long long)ticket);
+    if (pops && r.pops.empty())
+      return err(LBF_ESTATE, "populations were not kept (LBF_SERVE_KEEP_POPS)");
+    if (!release) {
+      // the request stays visible to other callers: copy under the lock
+      if (rho) memcpy(rho, r.rho.data(), r.rho.size() * 8);
+      if (u) memcpy(u, r.u.data(), r.u.size() * 8);
+      if (pops) memcpy(pops, r.pops.data(), r.pops.size() * 8);
+      return 0;
+    }
+    // a DONE request is no longer touched by the worker or the helpers, so
+    // once it leaves the table the copies need not hold the server lock
+    owned = std::move(it->second);
+    s->reqs.erase(it);
+  }
+  const Request& r = *owned;
   if (rho) memcpy(rho, r.rho.data(), r.rho.size() * 8);
   if (u) memcpy(u, r.u.data(), r.u.size() * 8);
-  if (pops) {
-    if (r.pops.empty()) return err(LBF_ESTATE, "populations were not kept (LBF_SERVE_KEEP_POPS)");
-    memcpy(pops, r.pops.data(), r.pops.size() * 8);
-  }
-  if (release) s->reqs.erase(it);
+  if (pops) memcpy(pops, r.pops.data(), r.pops.size() * 8);
   return 0;
 }
 
