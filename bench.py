@@ -277,6 +277,7 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
         "gpu_busy_frac": round(st["gpu_seconds"] / wall, 3),
         "hbm_peak_bytes": int(st["hbm_peak"]),
         "tuned_grid": int(st["tuned_grid"]),
+        "tuned_tile": int(st["tuned_tile"]),
         "poisson_rate_per_s": round(rate, 2),
         "latency_cases": int(len(lat_idx)),
         "served_p50_s": round(float(np.percentile(lat, 50)), 4),

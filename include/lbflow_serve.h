@@ -42,11 +42,14 @@ typedef struct lbf_server_config {
   uint64_t hbm_budget_bytes;    /* device bytes resident cases may hold (0 = 80 % of
                                    the device's free memory at creation)             */
   uint32_t max_steps_per_round; /* steps per graph replay, power of two <= 256 (0 = 64) */
-  int32_t autotune;             /* 1: choose the batched launch's workgroup count by
-                                   measurement per mix of case types                */
+  int32_t autotune;             /* 1: choose the batched launch's workgroup count and
+                                   tile size by measurement per mix of case types;
+                                   finished choices are kept for the process and
+                                   reused by later servers                          */
   uint32_t grid;                /* workgroups per batched launch when autotune = 0
                                    (0 = 2048)                                       */
-  uint32_t reserved;
+  uint32_t tile;                /* nodes per tile (= threads per workgroup) when
+                                   autotune = 0: 128, 256 or 512 (0 = 256)          */
 } lbf_server_config;
 
 /* submit flags */
@@ -82,7 +85,7 @@ typedef struct lbf_server_stats {
   double prep_seconds;        /* case creation + initialisation, summed over helper threads */
   double retire_seconds;      /* result download + release, summed over helper threads */
   uint32_t tuned_grid;        /* workgroup count the autotuner settled on last    */
-  uint32_t reserved;
+  uint32_t tuned_tile;        /* tile size the autotuner settled on last          */
 } lbf_server_stats;
 
 int lbf_server_create(const lbf_server_config* cfg, lbf_server** out);

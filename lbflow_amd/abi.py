@@ -128,7 +128,7 @@ REQ_QUEUED, REQ_PREPARING, REQ_RUNNING, REQ_DONE, REQ_FAILED = range(5)
 class ServerConfig(C.Structure):
     _fields_ = [("abi_version", C.c_uint32), ("max_slots", C.c_uint32),
                 ("hbm_budget_bytes", C.c_uint64), ("max_steps_per_round", C.c_uint32),
-                ("autotune", C.c_int32), ("grid", C.c_uint32), ("reserved", C.c_uint32)]
+                ("autotune", C.c_int32), ("grid", C.c_uint32), ("tile", C.c_uint32)]
 
 
 class RequestInfo(C.Structure):
@@ -148,7 +148,7 @@ class ServerStats(C.Structure):
                 ("mean_latency", C.c_double), ("cases_per_second", C.c_double),
                 ("mlups", C.c_double), ("prep_seconds", C.c_double),
                 ("retire_seconds", C.c_double), ("tuned_grid", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("tuned_tile", C.c_uint32)]
 
 
 # benchmark and tuning hooks (include/lbflow_tuning.h; no reference counterpart)

@@ -21,7 +21,8 @@ struct Launchers {
   LaunchFn aa_step = nullptr;   // in-place AA-pattern step (momentum only)
   LaunchFn aa_init = nullptr;
   LaunchFn aa_readback = nullptr;
-  ServeLaunchFn serve[2] = {nullptr, nullptr};  // BGK, TRT (D2Q9 / D3Q19, AA storages)
+  // [BGK, TRT][tile size 128, 256, 512] (D2Q9 / D3Q19, AA storages)
+  ServeLaunchFn serve[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
 };
 
 // precision: LBF_F64 .. LBF_BF16_SHIFTED; returns false when not built

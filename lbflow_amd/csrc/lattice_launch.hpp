@@ -68,8 +68,13 @@ static hipError_t aa_init_t(const KArgs& a, int grid, hipStream_t s) {
 template <class L, class St>
 static void serve_launchers(Launchers* out) {
   if constexpr (L::id == 1 || L::id == 3) {
-    out->serve[0] = serve_launch<L, St, 0>;
-    out->serve[1] = serve_launch<L, St, 1>;
+    static_assert(kNumTileSizes == 3, "one launcher per tile size");
+    out->serve[0][0] = serve_launch<L, St, 0, kTileSizes[0]>;
+    out->serve[0][1] = serve_launch<L, St, 0, kTileSizes[1]>;
+    out->serve[0][2] = serve_launch<L, St, 0, kTileSizes[2]>;
+    out->serve[1][0] = serve_launch<L, St, 1, kTileSizes[0]>;
+    out->serve[1][1] = serve_launch<L, St, 1, kTileSizes[1]>;
+    out->serve[1][2] = serve_launch<L, St, 1, kTileSizes[2]>;
   }
 }
 

@@ -60,6 +60,12 @@ constexpr int kHelpers = 8;  // threads that set up and release cases
 constexpr uint32_t kGridCandidates[] = {512, 1024, 2048, 4096, 8192};
 constexpr int kNumCandidates = sizeof(kGridCandidates) / sizeof(kGridCandidates[0]);
 
+int tile_index_of(uint32_t tile) {
+  for (int i = 0; i < kNumTileSizes; i++)
+    if ((uint32_t)kTileSizes[i] == tile) return i;
+  return -1;
+}
+
 int type_index(int vs, int prec, int op) {
   if (vs != LBF_D2Q9 && vs != LBF_D3Q19) return -1;
   if (prec < LBF_F32 || prec > LBF_BF16_SHIFTED) return -1;
@@ -84,7 +90,7 @@ struct Request {
   std::vector<uint8_t> node_type;
   std::vector<double> rho0, u0, force;
   uint64_t nodes = 0, bytes = 0;
-  int d = 2, q = 9, type = -1, blocks = 0;
+  int d = 2, q = 9, type = -1;
   int state = LBF_REQ_QUEUED, error = 0;
   std::string errmsg;
   double t_submit = 0, t_admit = 0, t_start = 0, t_done = 0;
@@ -94,12 +100,22 @@ struct Request {
   std::vector<double> rho, u, pops;
 };
 
+// Launch shape search for one mix of case types: first the workgroup counts at
+// the default tile size, then the other tile sizes at the best count; each
+// candidate is one round, scored as seconds per node update.
 struct Tune {
+  int phase = 0;  // 0: grid candidates, 1: tile candidates
   int next = 0;
   double best = 1e300;
   uint32_t best_grid = 2048;
+  int best_tile = kDefaultTileIdx;
   bool done = false;
 };
+
+// Finished searches, shared by every server of the process: (device, mix of
+// case types) -> (workgroups, tile size index).
+std::mutex g_tune_mu;
+std::map<uint64_t, std::pair<uint32_t, int>> g_tune_cache;
 
 }  // namespace
 }  // namespace lbf
@@ -133,7 +149,7 @@ struct lbf_server {
   double node_updates = 0, first_submit = -1, last_done = 0, gpu_seconds = 0;
   double prep_seconds = 0, retire_seconds = 0;
   std::vector<double> latencies;
-  uint32_t tuned_grid = 0;
+  uint32_t tuned_grid = 0, tuned_tile = 0;
   int resident = 0;
 
   // worker-owned
@@ -146,7 +162,9 @@ struct lbf_server {
   size_t tiles_cap[kTypes] = {};
   uint32_t* h_tiles = nullptr;
   size_t h_tiles_cap = 0;
-  ServeLaunchFn launch[kTypes] = {};
+  ServeLaunchFn launch[kTypes][kNumTileSizes] = {};
+  int cfg_tile = kDefaultTileIdx;  // tile size index when autotune is off
+  int cur_tile = kDefaultTileIdx;  // tile size index of the tile lists on the device
   bool dirty = true;
   std::map<uint64_t, hipGraphExec_t> graphs;
   std::map<uint32_t, Tune> tune;
@@ -247,7 +265,6 @@ struct lbf_server {
         finish_locked(r, LBF_REQ_FAILED, rc);
       } else {
         r->c = c;
-        r->blocks = info.grid;
         ready.push_back(r);
         cv_work.notify_one();
       }
@@ -289,11 +306,17 @@ struct lbf_server {
     graphs.clear();
   }
 
+  // tiles of a case at the current tile size (the last one may be partial)
+  uint32_t tiles_of(const Request* r) const {
+    const uint64_t t = (uint64_t)kTileSizes[cur_tile];
+    return (uint32_t)((r->nodes + t - 1) / t);
+  }
+
   int upload() {
     size_t total = 0;
     std::vector<size_t> count(kTypes, 0);
     for (int i = 0; i < kServeMaxSlots; i++)
-      if (slot_req[i]) count[slot_req[i]->type] += (size_t)slot_req[i]->blocks;
+      if (slot_req[i]) count[slot_req[i]->type] += (size_t)tiles_of(slot_req[i]);
     for (int t = 0; t < kTypes; t++) total += count[t];
     if (total > h_tiles_cap) {
       if (h_tiles) (void)hipHostFree(h_tiles);
@@ -320,7 +343,8 @@ struct lbf_server {
       for (int i = 0; i < kServeMaxSlots; i++) {
         Request* r = slot_req[i];
         if (!r || r->type != t) continue;
-        for (int b = 0; b < r->blocks; b++) h_tiles[k++] = ((uint32_t)i << (32 - kServeSlotBits)) | (uint32_t)b;
+        const uint32_t nb = tiles_of(r);
+        for (uint32_t b = 0; b < nb; b++) h_tiles[k++] = ((uint32_t)i << (32 - kServeSlotBits)) | b;
       }
       h_ntiles[t] = (uint32_t)count[t];
       if (count[t]) {
@@ -339,8 +363,9 @@ struct lbf_server {
     return 0;
   }
 
-  int get_graph(uint32_t mask, int K, uint32_t grid, hipGraphExec_t* out) {
-    const uint64_t key = (uint64_t)mask | ((uint64_t)K << 16) | ((uint64_t)grid << 32);
+  int get_graph(uint32_t mask, int K, uint32_t grid, int tile, hipGraphExec_t* out) {
+    const uint64_t key =
+        (uint64_t)mask | ((uint64_t)K << 16) | ((uint64_t)grid << 32) | ((uint64_t)tile << 56);
     auto it = graphs.find(key);
     if (it != graphs.end()) {
       *out = it->second;
@@ -370,7 +395,7 @@ struct lbf_server {
     for (int k = 0; k < K && ok; k++) {
       for (int t = 0; t < kTypes && ok; t++)
         if (mask >> t & 1u)
-          ok = launch[t](d_slots, d_tiles[t], d_ntiles + t, (int)grid, stream) == hipSuccess;
+          ok = launch[t][tile](d_slots, d_tiles[t], d_ntiles + t, (int)grid, stream) == hipSuccess;
       if (ok) {
         hipLaunchKernelGGL(serve_advance, dim3(1), dim3(kServeMaxSlots), 0, stream, d_slots);
         ok = hipGetLastError() == hipSuccess;
@@ -395,27 +420,63 @@ struct lbf_server {
     return 0;
   }
 
-  uint32_t choose_grid(uint32_t mask, bool* measuring) {
+  // Launch shape (workgroups, tile size index) for the next round of this mix.
+  void choose_shape(uint32_t mask, bool* measuring, uint32_t* grid, int* tile) {
     *measuring = false;
-    if (!cfg.autotune) return cfg.grid;
-    Tune& t = tune[mask];
-    if (t.done) return t.best_grid;
+    if (!cfg.autotune) {
+      *grid = cfg.grid;
+      *tile = cfg_tile;
+      return;
+    }
+    auto it = tune.find(mask);
+    if (it == tune.end()) {
+      Tune t;
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      auto c = g_tune_cache.find(((uint64_t)device << 32) | mask);
+      if (c != g_tune_cache.end()) {
+        t.done = true;
+        t.best_grid = c->second.first;
+        t.best_tile = c->second.second;
+      }
+      it = tune.emplace(mask, t).first;
+    }
+    Tune& t = it->second;
+    if (t.done) {
+      *grid = t.best_grid;
+      *tile = t.best_tile;
+      return;
+    }
     *measuring = true;
-    return kGridCandidates[t.next];
+    if (t.phase == 0) {
+      *grid = kGridCandidates[t.next];
+      *tile = kDefaultTileIdx;
+    } else {
+      *grid = t.best_grid;
+      *tile = t.next < kDefaultTileIdx ? t.next : t.next + 1;  // the other tile sizes
+    }
   }
 
-  void record_tuning(uint32_t mask, uint32_t grid, double seconds_per_update) {
+  void record_tuning(uint32_t mask, uint32_t grid, int tile, double seconds_per_update) {
     Tune& t = tune[mask];
     if (t.done) return;
     if (seconds_per_update < t.best) {
       t.best = seconds_per_update;
       t.best_grid = grid;
+      t.best_tile = tile;
     }
-    if (++t.next >= kNumCandidates) {
-      t.done = true;
-      std::lock_guard<std::mutex> lk(mu);
-      tuned_grid = t.best_grid;
+    if (t.phase == 0) {
+      if (++t.next >= kNumCandidates) t.phase = 1, t.next = 0;
+      return;
     }
+    if (++t.next < kNumTileSizes - 1) return;
+    t.done = true;
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tune_cache[((uint64_t)device << 32) | mask] = {t.best_grid, t.best_tile};
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    tuned_grid = t.best_grid;
+    tuned_tile = (uint32_t)kTileSizes[t.best_tile];
   }
 
   // Detaches a finished (or failed) case from its slot; a helper thread
@@ -531,6 +592,15 @@ struct lbf_server {
           max_rem = std::max(max_rem, slot_req[i]->remaining);
         }
       if (!n_res) continue;
+      const uint32_t mask = type_mask();
+      bool measuring = false;
+      uint32_t grid = 0;
+      int tile = kDefaultTileIdx;
+      choose_shape(mask, &measuring, &grid, &tile);
+      if (tile != cur_tile) {
+        cur_tile = tile;  // tile lists are per tile size
+        dirty = true;
+      }
       int rc = dirty ? upload() : 0;
       if (rc) {
         fail_all_resident(rc);
@@ -544,11 +614,8 @@ struct lbf_server {
       for (int i = 0; i < kServeMaxSlots; i++)
         if (slot_req[i])
           work += (double)slot_req[i]->nodes * (double)std::min<long long>(K, slot_req[i]->remaining);
-      const uint32_t mask = type_mask();
-      bool measuring = false;
-      const uint32_t grid = choose_grid(mask, &measuring);
       hipGraphExec_t g = nullptr;
-      if ((rc = get_graph(mask, K, grid, &g))) {
+      if ((rc = get_graph(mask, K, grid, tile, &g))) {
         fail_all_resident(rc);
         continue;
       }
@@ -562,7 +629,7 @@ struct lbf_server {
       }
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, ev0, ev1);
-      if (measuring) record_tuning(mask, grid, ms * 1e-3 / work);
+      if (measuring) record_tuning(mask, grid, tile, ms * 1e-3 / work);
       {
         std::lock_guard<std::mutex> lk(mu);
         rounds++;
@@ -651,6 +718,10 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
   if (!s->cfg.max_steps_per_round) s->cfg.max_steps_per_round = 64;
   s->cfg.max_steps_per_round = std::min<uint32_t>(s->cfg.max_steps_per_round, 256);
   if (!s->cfg.grid) s->cfg.grid = 2048;
+  if (!s->cfg.tile) s->cfg.tile = (uint32_t)kTileSizes[kDefaultTileIdx];
+  s->cfg_tile = tile_index_of(s->cfg.tile);
+  if (s->cfg_tile < 0) return err(LBF_EINVAL, "tile %u: use 128, 256 or 512", s->cfg.tile);
+  s->cur_tile = s->cfg.autotune ? kDefaultTileIdx : s->cfg_tile;
   if (hipGetDevice(&s->device) != hipSuccess) return err(LBF_EDEVICE, "hipGetDevice failed");
   if (!s->cfg.hbm_budget_bytes) {
     size_t free_b = 0, total_b = 0;
@@ -673,8 +744,10 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
       const int lat = (t / 2) / 4, prec = LBF_F32 + (t / 2) % 4, op = t % 2;
       Launchers L{};
       const bool have = lat ? launchers_d3q19(prec, &L) : launchers_d2q9(prec, &L);
-      ok = ok && have && L.serve[op];
-      if (ok) s->launch[t] = L.serve[op];
+      for (int k = 0; k < kNumTileSizes; k++) {
+        ok = ok && have && L.serve[op][k];
+        if (ok) s->launch[t][k] = L.serve[op][k];
+      }
     }
     ok = ok && hipMemcpy(s->d_slots, s->h_slots, sizeof(ServeSlot) * kServeMaxSlots,
                          hipMemcpyHostToDevice) == hipSuccess &&
@@ -861,6 +934,7 @@ int lbf_server_get_stats(lbf_server* s, lbf_server_stats* o) {
   o->prep_seconds = s->prep_seconds;
   o->retire_seconds = s->retire_seconds;
   o->tuned_grid = s->tuned_grid;
+  o->tuned_tile = s->tuned_tile;
   return 0;
 }
 

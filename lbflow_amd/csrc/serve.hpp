@@ -5,9 +5,10 @@
 // the step index whose launch starts from natural slots. A tile is one
 // workgroup's worth of nodes of one case: (slot << 24) | block. One launch per
 // case type (lattice x storage x operator) walks that type's tile list with a
-// fixed, tunable number of workgroups, so the launch shape does not depend on
-// which cases are resident and a captured graph of K steps stays valid while
-// cases come and go: admission and retirement only rewrite the tables.
+// fixed, tunable number of workgroups of a tunable tile size (TILE nodes, one
+// per thread), so the launch shape does not depend on which cases are
+// resident and a captured graph of K steps stays valid while cases come and
+// go: admission and retirement only rewrite the tables.
 //
 // Each tile runs exactly the per-node code of the single-case AA kernel
 // (aa_node), with the case's own parity and fresh flag read from its slot, so
@@ -24,6 +25,10 @@ namespace lbf {
 constexpr int kServeSlotBits = 8;
 constexpr int kServeMaxSlots = 1 << kServeSlotBits;
 constexpr uint32_t kServeBlockMask = (1u << (32 - kServeSlotBits)) - 1;
+// tile sizes (nodes per tile = threads per workgroup) the autotuner picks from
+constexpr int kNumTileSizes = 3;
+constexpr int kTileSizes[kNumTileSizes] = {128, 256, 512};
+constexpr int kDefaultTileIdx = 1;  // 256, the single-case kernels' workgroup
 
 struct ServeSlot {
   KArgs a;                        // single-case step arguments (flags: store moments only)
@@ -31,8 +36,8 @@ struct ServeSlot {
   unsigned long long fresh_step;  // this step index reads natural slots (~0: none)
 };
 
-template <class L, class St, int OP>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+template <class L, class St, int OP, int TILE>
+__global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5)))
     serve_step(const ServeSlot* __restrict__ slots, const uint32_t* __restrict__ tiles,
                const uint32_t* __restrict__ ntiles) {
   const uint32_t nt = *ntiles;
@@ -44,7 +49,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     if (rem <= 0) continue;  // finished within this graph replay
     const unsigned long long step = s.a.ctrl->step;
     const bool fresh = step == s.fresh_step;
-    const uint32_t n = (tile & kServeBlockMask) * kBlock + threadIdx.x;
+    const uint32_t n = (tile & kServeBlockMask) * TILE + threadIdx.x;
     if (rem == 1) {
       if (step & 1)
         aa_node<L, St, OP, true, true>(s.a, n, step, fresh, acc);
@@ -59,10 +64,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
   }
 }
 
-template <class L, class St, int OP>
+template <class L, class St, int OP, int TILE>
 hipError_t serve_launch(const ServeSlot* slots, const uint32_t* tiles, const uint32_t* ntiles,
                         int grid, hipStream_t s) {
-  hipLaunchKernelGGL((serve_step<L, St, OP>), dim3(grid), dim3(kBlock), 0, s, slots, tiles,
+  hipLaunchKernelGGL((serve_step<L, St, OP, TILE>), dim3(grid), dim3(TILE), 0, s, slots, tiles,
                      ntiles);
   return hipGetLastError();
 }

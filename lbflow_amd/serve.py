@@ -25,12 +25,12 @@ _dp = C.POINTER(C.c_double)
 
 class Server:
     def __init__(self, hbm_budget_bytes: int = 0, max_slots: int = 0, max_steps_per_round: int = 0,
-                 autotune: bool = True, grid: int = 0, lib=None):
+                 autotune: bool = True, grid: int = 0, tile: int = 0, lib=None):
         self.lib = lib if lib is not None else load_library()
         cfg = abi.ServerConfig(abi_version=abi.ABI_VERSION, max_slots=max_slots,
                                hbm_budget_bytes=hbm_budget_bytes,
                                max_steps_per_round=max_steps_per_round,
-                               autotune=1 if autotune else 0, grid=grid)
+                               autotune=1 if autotune else 0, grid=grid, tile=tile)
         h = C.c_void_p()
         self._call(self.lib.lbf_server_create, C.byref(cfg), C.byref(h))
         self.h = h

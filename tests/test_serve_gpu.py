@@ -1,10 +1,13 @@
 """Serving tier on the GPU: a served case equals the single-case path, each
-result is bitwise the same alone and inside a mix, admission keeps the
-resident bytes within the HBM budget, and unservable cases are refused."""
+result is bitwise the same alone and inside a mix and at every tile size,
+the autotuner searches workgroup counts and tile sizes and caches its
+choice, admission keeps the resident bytes within the HBM budget, and
+unservable cases are refused."""
 import numpy as np
 import pytest
 
 from lbflow_amd import abi, configs
+from lbflow_amd.case import Collision
 from lbflow_amd.engine import Case, LbfError
 from lbflow_amd.serve import Server
 
@@ -72,6 +75,58 @@ def test_result_is_bitwise_the_same_alone_and_in_a_mix():
             assert np.isfinite(u).all()
     for a, b in zip(alone, mixed):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("tile", [128, 256, 512])
+def test_each_tile_size_gives_the_single_case_result(tile):
+    # 37^2 and 19^3 nodes leave a partial last tile at every size
+    specs = [(case(abi.D2Q9, 37, abi.F32, seed=21), 13),
+             (case(abi.D3Q19, 19, abi.F16_SHIFTED, seed=22), 11)]
+    print(f"[serve-tile] tile={tile}", flush=True)
+    refs = [single(sp, n) for sp, n in specs]
+    with Server(autotune=False, tile=tile, max_steps_per_round=4) as srv:
+        ts = [srv.submit(sp, n, keep_pops=True) for sp, n in specs]
+        for t, ref in zip(ts, refs):
+            srv.wait(t, 120)
+            got = srv.result(t)
+            for a, b in zip(ref, got):
+                assert np.array_equal(a, b)
+
+
+def test_a_tile_size_outside_the_set_is_refused():
+    with pytest.raises(LbfError) as e:
+        Server(autotune=False, tile=100)
+    assert e.value.code == abi.EINVAL
+
+
+def test_autotuner_searches_tile_sizes_and_caches_its_choice():
+    # D2Q9 TRT in BF16-shifted storage: a mix of case types no other test
+    # serves, so the first server starts untuned (choices last for the process)
+    def trt(seed):
+        return case(abi.D2Q9, 48, abi.BF16_SHIFTED, seed=seed).replace(
+            collision=Collision.trt(1.25, 8.0 / 9.0))
+    specs = [(trt(30 + i), 12) for i in range(3)]
+    refs = [single(sp, n) for sp, n in specs]
+    print("[serve-tune] searching", flush=True)
+    with Server(max_steps_per_round=1) as srv:  # one step per round
+        ts = [srv.submit(sp, n, keep_pops=True) for sp, n in specs]
+        srv.drain(120)
+        st = srv.stats()
+        got = [srv.result(t) for t in ts]
+    # five workgroup counts at 256-node tiles, then the 128 and 512 tiles
+    assert st["tuned_grid"] in (512, 1024, 2048, 4096, 8192)
+    assert st["tuned_tile"] in (128, 256, 512)
+    assert st["graphs_built"] >= 7
+    # the launch shape changed between rounds; the results did not
+    for ref, g in zip(refs, got):
+        for a, b in zip(ref, g):
+            assert np.array_equal(a, b)
+    print("[serve-tune] cached", flush=True)
+    with Server(max_steps_per_round=1) as srv:
+        t = srv.submit(specs[0][0], specs[0][1])
+        srv.wait(t, 60)
+        st2 = srv.stats()
+    assert st2["graphs_built"] == 1 and st2["tuned_tile"] == 0  # no search
 
 
 def test_admission_respects_the_hbm_budget():
