@@ -10,7 +10,9 @@ timed in f64 (the parity path) and in the shifted FP16 storage tier when it
 is built, and reported under "extra". The roofline block uses algorithmic
 bytes 2*Q*s per node update; the CPU baseline times the oracle (the
 line-by-line f64 restatement of the reference solver) on the host cores for
-a bounded sample of the same workload.
+a bounded sample of the same workload. The same kernel on an all-fluid, fully
+periodic 256^3 box (uniform initial velocity 0.05 along x, no bounce-back
+links) is reported as extra.all_fluid_periodic: the clean roofline point.
 """
 from __future__ import annotations
 
@@ -380,6 +382,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default="", help="comma list of variants (f32,f64,f16s,...)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix leg")
+    ap.add_argument("--no-all-fluid", action="store_true",
+                    help="skip the all-fluid periodic box (the kernel without bounce-back)")
     ap.add_argument("--serve-cases", type=int, default=2000)
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-tier report (errors vs FP64 per configuration)")
@@ -431,6 +435,24 @@ def main():
                              streaming="aa" if streaming == abi.STREAM_AA else "two-array")
         print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)",
               file=sys.stderr, flush=True)
+    # the same kernel on an all-fluid, fully periodic box: no bounce-back links,
+    # so this isolates the streaming pattern from the porous geometry
+    all_fluid = {}
+    if not args.no_all_fluid:
+        for name, prec, sbytes in (("f32", abi.F32, 4), ("f32s", abi.F32_SHIFTED, 4),
+                                   ("f16s", abi.F16_SHIFTED, 2)):
+            sp = configs.periodic_box(n=n, precision=prec, streaming=abi.STREAM_AA).replace(
+                initial_velocity=(0.05, 0.0, 0.0))
+            ms, wall, nonfinite, mass = time_case(sp, args.steps, args.warmup, hip)
+            mlups = N * args.steps / (ms / 1e3) / 1e6
+            gbs = mlups * 1e6 * 2 * 19 * sbytes / 1e9
+            all_fluid[name] = dict(mlups=round(mlups, 1), ms_per_step=round(ms / args.steps, 4),
+                                   achieved_gbs=round(gbs, 1),
+                                   hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
+                                   frac_of_sweep=round(gbs / sweep_gbs[sbytes], 4),
+                                   nonfinite=nonfinite)
+            print(f"[bench] all-fluid periodic {n}^3 {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s "
+                  f"({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)", file=sys.stderr, flush=True)
     head = results.get("f32") or next(iter(results.values()))
     head_name = "f32" if "f32" in results else next(iter(results))
     serving = None if args.no_serve else serving_leg(count=args.serve_cases)
@@ -466,7 +488,8 @@ def main():
         "cpu_baseline": cpu,
         "serving": serving,
         "extra": dict(results, copy_gbs=round(copy_gbs, 1),
-                      sweep_gbs={f"{8 * es}-bit": round(g, 1) for es, g in sweep_gbs.items()}),
+                      sweep_gbs={f"{8 * es}-bit": round(g, 1) for es, g in sweep_gbs.items()},
+                      all_fluid_periodic=all_fluid or None),
     }
     print(json.dumps(line))
 
