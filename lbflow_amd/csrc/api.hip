@@ -307,19 +307,28 @@ void to_compute(const double* in, size_t n, std::vector<char>& out) {
   for (size_t i = 0; i < n; i++) o[i] = static_cast<T>(in[i]);
 }
 
+// Host <-> device copy in the case's stream order, waited for. No copy goes
+// through the legacy null stream, so a case never waits for work on other
+// streams (a server's graph replays, other cases).
+hipError_t copy_sync(lbf_case* c, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+  hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+
 int upload_compute(lbf_case* c, void* dst, const double* src, size_t n) {
   std::vector<char> buf;
   if (c->tb == 8)
     to_compute<double>(src, n, buf);
   else
     to_compute<float>(src, n, buf);
-  HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
   return 0;
 }
 
 int download_compute(lbf_case* c, const void* src, double* out, size_t n) {
   std::vector<char> buf(n * c->tb);
-  HIPCHK(hipMemcpy(buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c, buf.data(), src, buf.size(), hipMemcpyDeviceToHost));
   if (c->tb == 8)
     memcpy(out, buf.data(), n * 8);
   else
@@ -334,7 +343,7 @@ int sync_and_check(lbf_case* c) {
 
 int read_ctrl(lbf_case* c, Ctrl* h) {
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c, h, c->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost));
   c->host_step = h->step;
   return 0;
 }
@@ -603,7 +612,7 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     for (long long i = 0; i < N; i++)
       for (int a = 0; a < c->d; a++) F[a * c->ld + i] = d->force_field[i * c->d + a];
     if ((rc = alloc_sync((void**)&c->force, F.size() * 8))) return cleanup(rc);
-    if (hipMemcpy(c->force, F.data(), F.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+    if (copy_sync(c, c->force, F.data(), F.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return cleanup(fail(LBF_EDEVICE, "upload of force field failed"));
   }
   for (int s = 0; s < c->ns; s++) {
@@ -632,7 +641,7 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
       std::vector<double> sf(c->ld, 0.0);
       for (long long i = 0; i < N; i++) sf[i] = sd.source_field[i];
       if ((rc = alloc_sync((void**)&S.src, c->ld * 8))) return cleanup(rc);
-      if (hipMemcpy(S.src, sf.data(), c->ld * 8, hipMemcpyHostToDevice) != hipSuccess)
+      if (copy_sync(c, S.src, sf.data(), c->ld * 8, hipMemcpyHostToDevice) != hipSuccess)
         return cleanup(fail(LBF_EDEVICE, "upload of scalar source failed"));
     }
   }
@@ -913,7 +922,7 @@ int lbf_reduce(lbf_case* c, int32_t which, double* out) {
       return fail(LBF_EINVAL, "unknown reduction %d", which);
   }
   HIPCHK(hipStreamSynchronize(c->stream));
-  HIPCHK(hipMemcpy(&r, c->red_out, 8, hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c, &r, c->red_out, 8, hipMemcpyDeviceToHost));
   if (kind == LBF_REDUCE_MEAN_DENSITY) r = r / (double)c->n_fluid;
   *out = r;
   return 0;
@@ -953,7 +962,7 @@ int lbf_get_scalar(lbf_case* c, uint32_t s, double* value, double* q_ads) {
   }
   if (q_ads) {
     if (c->sc[s].q_ads) {
-      HIPCHK(hipMemcpy(q_ads, c->sc[s].q_ads, c->N * 8, hipMemcpyDeviceToHost));
+      HIPCHK(copy_sync(c, q_ads, c->sc[s].q_ads, c->N * 8, hipMemcpyDeviceToHost));
     } else {
       for (long long i = 0; i < c->N; i++) q_ads[i] = NAN;
     }
@@ -984,7 +993,7 @@ int lbf_get_pops(lbf_case* c, int32_t field, double* out) {
     HIPCHK(c->L.sca_step(a, c->grid, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   std::vector<double> soa((size_t)c->q * c->ld);
-  HIPCHK(hipMemcpy(soa.data(), c->readback, bytes, hipMemcpyDeviceToHost));
+  HIPCHK(copy_sync(c, soa.data(), c->readback, bytes, hipMemcpyDeviceToHost));
   for (long long i = 0; i < c->N; i++)
     for (int j = 0; j < c->q; j++) out[i * c->q + j] = soa[(size_t)j * c->ld + i];
   return 0;
@@ -1016,7 +1025,7 @@ int lbf_set_pops(lbf_case* c, int32_t field, const double* in) {
     float* o = reinterpret_cast<float*>(buf.data());
     for (size_t i = 0; i < soa.size(); i++) o[i] = (float)soa[i];
   }
-  HIPCHK(hipMemcpy(dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
+  HIPCHK(copy_sync(c, dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
   HIPCHK(launch_ctrl_set(c->ctrl, kCtrlFresh, 1ll << field, c->stream));
   if (field == 0) c->aa_fresh = true;
   return 0;
@@ -1029,7 +1038,7 @@ int lbf_get_link_tables(lbf_case* c, uint8_t* node_type, uint32_t* bb_link_mask,
   if (bb_link_mask || face_bits) {
     std::vector<uint32_t> cls(c->ld);
     HIPCHK(hipStreamSynchronize(c->stream));
-    HIPCHK(hipMemcpy(cls.data(), c->cls, c->ld * 4, hipMemcpyDeviceToHost));
+    HIPCHK(copy_sync(c, cls.data(), c->cls, c->ld * 4, hipMemcpyDeviceToHost));
     const long long n3[3] = {c->nx, c->ny, c->nz};
     for (long long i = 0; i < c->N; i++) {
       const bool fluid = !(cls[i] & kSolidBit);

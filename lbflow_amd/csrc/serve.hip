@@ -681,6 +681,20 @@ struct lbf_server {
 
 namespace {
 
+// The worker's stream is created with a CU mask that enables every CU. HIP
+// gives such a stream a hardware queue of its own instead of one of the
+// pooled queues, so the helpers' set-up and release work, which runs on the
+// cases' own streams, does not wait behind a graph replay that happens to
+// share its queue.
+hipError_t create_worker_stream(int device, hipStream_t* out) {
+  int cus = 0;
+  hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+  if (e != hipSuccess) return e;
+  if (cus <= 0) return hipErrorInvalidValue;
+  std::vector<uint32_t> mask(((size_t)cus + 31) / 32, 0xffffffffu);
+  return hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data());
+}
+
 int check_servable(const lbf_case_desc* d, int* type) {
   if (!d) return err(LBF_EINVAL, "descriptor is NULL");
   if (d->abi_version != LBF_ABI_VERSION) return err(LBF_EINVAL, "ABI version %u", d->abi_version);
@@ -729,7 +743,7 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
     s->cfg.hbm_budget_bytes = (uint64_t)(0.8 * (double)free_b);
   }
   s->t0 = std::chrono::steady_clock::now();
-  bool ok = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) == hipSuccess &&
+  bool ok = create_worker_stream(s->device, &s->stream) == hipSuccess &&
             hipMalloc((void**)&s->d_slots, sizeof(ServeSlot) * kServeMaxSlots) == hipSuccess &&
             hipMalloc((void**)&s->d_ntiles, kTypes * 4) == hipSuccess &&
             hipHostMalloc((void**)&s->h_slots, sizeof(ServeSlot) * kServeMaxSlots) == hipSuccess &&
@@ -749,9 +763,12 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
         if (ok) s->launch[t][k] = L.serve[op][k];
       }
     }
-    ok = ok && hipMemcpy(s->d_slots, s->h_slots, sizeof(ServeSlot) * kServeMaxSlots,
-                         hipMemcpyHostToDevice) == hipSuccess &&
-         hipMemcpy(s->d_ntiles, s->h_ntiles, kTypes * 4, hipMemcpyHostToDevice) == hipSuccess;
+    ok = ok &&
+         hipMemcpyAsync(s->d_slots, s->h_slots, sizeof(ServeSlot) * kServeMaxSlots,
+                        hipMemcpyHostToDevice, s->stream) == hipSuccess &&
+         hipMemcpyAsync(s->d_ntiles, s->h_ntiles, kTypes * 4, hipMemcpyHostToDevice, s->stream) ==
+             hipSuccess &&
+         hipStreamSynchronize(s->stream) == hipSuccess;
   }
   if (!ok) {
     s->shutdown();
