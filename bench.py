@@ -165,8 +165,44 @@ def cpu_baseline(n=128, steps=450, threads=None):
                       f"({dt:.1f} s)"}
 
 
+def serving_cpu_baseline(params, spec_of, nodes_steps, budget_s=15.0, seed=7):
+    """The oracle (C f64 restatement of the reference step, OpenMP on all host
+    cores) on the serving mix: cases taken in a seeded random order and run
+    one after another (create, initialise, steps, fields) until budget_s has
+    passed, then scaled to the whole mix by node updates."""
+    import numpy as np
+    from lbflow_amd import abi
+    from oracle import oracle
+    lib = oracle.load()
+    cores = lib.lbo_max_threads()
+    order = np.random.default_rng(seed).permutation(len(params))
+    t_sum = 0.0
+    done_updates = 0.0
+    ran = 0
+    for i in order:
+        p = params[i]
+        sp = spec_of(p, abi.F64)
+        t0 = time.perf_counter()
+        o = oracle.OracleCase(sp)
+        o.initialize_nodes()
+        o.main_steps(p["steps"])
+        o.fields()
+        o.close()
+        t_sum += time.perf_counter() - t0
+        done_updates += sp.num_nodes * p["steps"]
+        ran += 1
+        if t_sum >= budget_s and ran >= 3:
+            break
+    total_s = t_sum * nodes_steps / done_updates
+    return {"cases_per_s": round(len(params) / total_s, 3),
+            "mlups": round(done_updates / t_sum / 1e6, 2), "cores": int(cores), "kind": "port",
+            "sample": f"oracle (C f64 restatement of the reference step, OpenMP) on {ran} cases "
+                      f"of the mix in seeded random order ({t_sum:.1f} s), scaled to the whole "
+                      f"mix by node updates"}
+
+
 def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100,
-                latency_sample=200, seed=5):
+                latency_sample=200, seed=5, cpu=True):
     """Serving mix on the batched server versus sequential plain f64 runs.
 
     Mix (configs.serving_mix): D2Q9 / D3Q19 50/50, sides log-uniform in
@@ -263,6 +299,11 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
             srv.result(t)
         st_lat = srv.stats()
     lat = np.array(lat)
+    cpu_base = None
+    if cpu:
+        print("[serve] CPU baseline: the oracle on a time-bounded sample of the mix",
+              file=sys.stderr, flush=True)
+        cpu_base = serving_cpu_baseline(params, spec_of, nodes_steps)
     return {
         "metric": "served cases/s and time to solution, serving mix",
         "cases": count,
@@ -286,6 +327,9 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
         "served_p99_s": round(float(np.percentile(lat, 99)), 4),
         "plain_p50_s": round(float(np.percentile(plain_lat, 50)), 4),
         "plain_p99_s": round(float(np.percentile(plain_lat, 99)), 4),
+        "cpu_baseline": cpu_base,
+        "throughput_ratio_vs_cpu": (round(served_cps / cpu_base["cases_per_s"], 1)
+                                    if cpu_base else None),
         "graphs_built_throughput": int(st["graphs_built"]),
         "graphs_built_latency": int(st_lat["graphs_built"]),
     }
@@ -455,7 +499,7 @@ def main():
                   f"({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)", file=sys.stderr, flush=True)
     head = results.get("f32") or next(iter(results.values()))
     head_name = "f32" if "f32" in results else next(iter(results))
-    serving = None if args.no_serve else serving_leg(count=args.serve_cases)
+    serving = None if args.no_serve else serving_leg(count=args.serve_cases, cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline()
     bytes_per_launch = N * head["bytes_per_site"]
     from lbflow_amd.profiling import read_traffic
