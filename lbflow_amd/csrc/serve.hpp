@@ -42,7 +42,17 @@ __global__ void __launch_bounds__(TILE) __attribute__((amdgpu_waves_per_eu(5)))
                const uint32_t* __restrict__ ntiles) {
   const uint32_t nt = *ntiles;
   double acc[5] = {0, 0, 0, 0, 0};
-  for (uint32_t t = blockIdx.x; t < nt; t += gridDim.x) {
+  // Workgroups are dealt round-robin to the 8 XCDs. When the grid is a
+  // multiple of 8 G, permute the ids within each group of 8 G so that every
+  // XCD walks runs of G consecutive tiles (neighbouring rows share its L2);
+  // the walk still visits each tile exactly once.
+  constexpr uint32_t G = 4;
+  uint32_t first = blockIdx.x;
+  if (gridDim.x % (8 * G) == 0) {
+    const uint32_t w = blockIdx.x >> 3;
+    first = ((w / G) * 8u + (blockIdx.x & 7u)) * G + w % G;
+  }
+  for (uint32_t t = first; t < nt; t += gridDim.x) {
     const uint32_t tile = tiles[t];
     const ServeSlot& s = slots[tile >> (32 - kServeSlotBits)];
     const long long rem = s.remaining;
