@@ -685,14 +685,19 @@ namespace {
 // gives such a stream a hardware queue of its own instead of one of the
 // pooled queues, so the helpers' set-up and release work, which runs on the
 // cases' own streams, does not wait behind a graph replay that happens to
-// share its queue.
+// share its queue. Such a stream is blocking: work on the legacy null stream
+// (none in this library) would wait for it. If the masked stream cannot be
+// created, a plain non-blocking stream serves.
 hipError_t create_worker_stream(int device, hipStream_t* out) {
   int cus = 0;
   hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-  if (e != hipSuccess) return e;
-  if (cus <= 0) return hipErrorInvalidValue;
-  std::vector<uint32_t> mask(((size_t)cus + 31) / 32, 0xffffffffu);
-  return hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data());
+  if (e == hipSuccess && cus > 0) {
+    std::vector<uint32_t> mask(((size_t)cus + 31) / 32, 0xffffffffu);
+    if (hipExtStreamCreateWithCUMask(out, (uint32_t)mask.size(), mask.data()) == hipSuccess)
+      return hipSuccess;
+    (void)hipGetLastError();  // clear the failed attempt's error
+  }
+  return hipStreamCreateWithFlags(out, hipStreamNonBlocking);
 }
 
 int check_servable(const lbf_case_desc* d, int* type) {
