@@ -150,6 +150,19 @@ def test_admission_respects_the_hbm_budget():
     assert peak <= 2
 
 
+def test_a_case_larger_than_the_budget_is_refused_at_submission():
+    from lbflow_amd.engine import device_bytes
+    spec = case(abi.D3Q19, 24, abi.F32, seed=6)
+    with Server(hbm_budget_bytes=device_bytes(spec) - 1) as srv:
+        with pytest.raises(LbfError) as e:
+            srv.submit(spec, 10)
+        assert e.value.code == abi.ENOMEM
+        small = case(abi.D2Q9, 16, abi.F32, seed=6)
+        t = srv.submit(small, 5)  # the server still serves what fits
+        srv.wait(t, 60)
+        assert srv.info(t).state == abi.REQ_DONE
+
+
 def test_unservable_cases_are_refused():
     with Server() as srv:
         refused = [
