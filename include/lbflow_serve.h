@@ -9,7 +9,8 @@
  *    budget and a slot is free; admission happens as earlier cases finish;
  *  - resident cases keep their populations in HBM and advance together in
  *    batched launches (one per case type) captured in HIP graphs of K steps,
- *    whose workgroup count an autotuner picks by measurement;
+ *    whose workgroup count and tile size an autotuner picks by measurement
+ *    (choices are kept for the process);
  *  - a finished case's fields (and, on request, populations) are copied to
  *    host memory and its device memory is freed.
  * A case advances exactly as through lbf_main_steps(case, steps, 0), so its
@@ -21,9 +22,10 @@
  * periodic; no scalars. Other descriptors are rejected at submit with
  * LBF_EUNSUPPORTED.
  *
- * Threading: a server owns a worker thread (the GPU loop), a preparation
- * thread (case creation and initialisation) and a finishing thread (result
- * download and release). All functions may be called from any thread.
+ * Threading: a server owns a worker thread (the GPU loop, on a stream with
+ * a hardware queue of its own) and eight helper threads (case creation and
+ * initialisation; result download and release). All functions may be called
+ * from any thread.
  */
 #ifndef LBFLOW_SERVE_H
 #define LBFLOW_SERVE_H
@@ -74,7 +76,7 @@ typedef struct lbf_request_info {
 typedef struct lbf_server_stats {
   uint64_t submitted, completed, failed, queued, resident;
   uint64_t rounds;            /* graph replays                                   */
-  uint64_t graphs_built;      /* distinct (type mix, K, grid) graphs captured     */
+  uint64_t graphs_built;      /* distinct (type mix, K, grid, tile) graphs captured */
   uint64_t hbm_in_use, hbm_peak, hbm_budget;
   double node_updates;        /* sum of nodes * steps over completed requests     */
   double first_submit, last_done; /* seconds since server creation               */
