@@ -212,7 +212,7 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
       - throughput: all `count` requests submitted at once, each result
         fetched as its case completes, served cases/s;
         the plain baseline runs a uniform random sample of `baseline_sample`
-        cases one after another (create, f64 two-array steps, fields) and
+        cases one after another (create, FP64 steps on the default path, fields) and
         scales its time to the whole mix;
       - latency: `latency_sample` cases arrive as a Poisson process at 1.2x
         the plain path's capacity; served time-to-solution is measured live,
@@ -238,7 +238,7 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
     def plain_service(idx):
         times = []
         for i in idx:
-            sp = spec_of(params[i], abi.F64).replace(streaming=abi.STREAM_TWO_ARRAY)
+            sp = spec_of(params[i], abi.F64)  # the default FP64 path
             t0 = time.perf_counter()
             c = Case(sp)
             c.initialize_nodes()

@@ -102,7 +102,10 @@ extern "C" {
 #define LBF_BF16_SHIFTED 4 /* (f_i - w_i)*scale stored in bfloat16             */
 
 /* ---- streaming scheme --------------------------------------------------- */
-#define LBF_STREAM_AUTO 0      /* AA for the fast tiers, two arrays for f64    */
+#define LBF_STREAM_AUTO 0      /* AA wherever the case allows it; in FP64 only
+                                  where it gives the two-array scheme's bits (not
+                                  with a Zou-He face next to another non-periodic
+                                  face); else two arrays                         */
 #define LBF_STREAM_TWO_ARRAY 1 /* fused pull-stream + collide, ping-pong       */
 #define LBF_STREAM_AA 2        /* in-place AA pattern (one population array)   */
 

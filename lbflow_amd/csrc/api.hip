@@ -104,6 +104,20 @@ bool aa_compatible(const lbf_case_desc* d, const VsInfo& v) {
   return true;
 }
 
+// The in-place scheme gives the reference's bits in FP64 unless a Zou-He face
+// meets another non-periodic face: at such edges the reference applies the
+// faces in order (src/lib.rs:27-29), so the Zou-He closure can still see the
+// wrapped pull values of a later face's links, while the in-place gather has
+// already put bounce-back values there.
+bool aa_reproduces_reference(const lbf_case_desc* d, const VsInfo& v) {
+  for (int f = 0; f < 2 * v.d; f++) {
+    if (d->faces[f].kind != LBF_BC_ZOU_HE) continue;
+    for (int g = 0; g < 2 * v.d; g++)
+      if (g / 2 != f / 2 && d->faces[g].kind != LBF_BC_PERIODIC) return false;
+  }
+  return true;
+}
+
 bool get_launchers(int vs, int prec, Launchers* L) {
   switch (vs) {
     case LBF_D2Q9: return launchers_d2q9(prec, L);
@@ -425,8 +439,11 @@ int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
   const long long ld = plane_stride(N);
   const size_t sb = storage_bytes(d->precision), tb = d->precision == LBF_F64 ? 8 : 4;
   const bool aa_ok = aa_compatible(d, v);
-  const bool aa = d->precision >= LBF_F32_SHIFTED || d->streaming == LBF_STREAM_AA ||
-                  (d->streaming == LBF_STREAM_AUTO && aa_ok && d->precision != LBF_F64);
+  const bool forced = d->precision >= LBF_F32_SHIFTED || d->streaming == LBF_STREAM_AA;
+  bool aa = forced || (d->streaming == LBF_STREAM_AUTO && aa_ok &&
+                       (d->precision != LBF_F64 || aa_reproduces_reference(d, v)));
+  // as lbf_create: the automatic choice falls back to two arrays at 4 GiB
+  if (aa && !forced && (unsigned long long)ld * v.q * sb >= (1ull << 32)) aa = false;
   uint64_t b = (aa ? 1ull : 2ull) * v.q * ld * sb + 4ull * ld + 2ull * (1 + v.d) * ld * tb;
   if (d->force_kind == LBF_SOURCE_FIELD) b += 8ull * v.d * ld;
   if (d->multiphase) b += (uint64_t)ld * tb;
@@ -473,7 +490,8 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
                                     "periodic, Zou-He})");
     }
     if (d->streaming == LBF_STREAM_AA) c->aa = true;
-    else if (d->streaming == LBF_STREAM_AUTO) c->aa = aa_ok && c->prec != LBF_F64;
+    else if (d->streaming == LBF_STREAM_AUTO)
+      c->aa = aa_ok && (c->prec != LBF_F64 || aa_reproduces_reference(d, v));
     if (shifted) c->aa = true;
     if (c->aa && (unsigned long long)c->ld * c->q * c->sb >= (1ull << 32)) {
       if (shifted || d->streaming == LBF_STREAM_AA) {

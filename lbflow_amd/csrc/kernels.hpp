@@ -398,6 +398,50 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
   }
 }
 
+// The parity kernel's arithmetic after streaming, for the in-place kernel
+// with FP64 storage (no Shan-Chen force there): the same calls in the same
+// order, compiled without contraction, so both schemes give the same bits.
+// Moments, then the body-force shift of u (src/momentum/node.rs:450-455).
+template <class L>
+__device__ __forceinline__ void ref_moments_force(const KArgs& a, uint32_t n, const double* f,
+                                                  double& rho, double* u, double* F) {
+  constexpr int D = L::D;
+  moments<L>(f, rho, u);
+  F[0] = F[1] = F[2] = 0.0;
+  if (a.force_kind) {
+    sfor<D>([&](auto A) {
+      constexpr int ax = decltype(A)::value;
+      F[ax] = a.force_kind == 1 ? a.F[ax] : a.src_field[(size_t)ax * (size_t)a.ld + n];
+    });
+    sfor<D>([&](auto A) {
+      constexpr int ax = decltype(A)::value;
+      u[ax] += 0.5 * F[ax] / rho;
+    });
+  }
+}
+
+// Equilibrium and collision (src/kernel.rs:3-120, src/momentum/node.rs:311-333).
+template <class L, int OP>
+__device__ __forceinline__ void ref_collide(const KArgs& a, const double* f, double rho,
+                                            const double* u, const double* F, double* fs) {
+  constexpr int Q = L::Q;
+  double feq[Q];
+  equilibrium<L>(rho, u, feq);
+  if constexpr (OP == 0) {
+    bgk<L>(f, feq, a.omega, a.omega_p, fs);
+    if (a.force_kind) add_guo<L>(u, F, a.coeff_b, fs);
+  } else if constexpr (OP == 1) {
+    trt<L>(f, feq, a.wp, a.wm, fs);
+  } else {
+    double s[Q];
+    sfor<Q>([&](auto KI) {
+      constexpr int kk = decltype(KI)::value;
+      s[kk] = a.s[kk];
+    });
+    mrt<L>(rho, u, f, feq, s, fs);
+  }
+}
+
 // ------------------------------------------------------------- scalar
 // Gathers the post-stream, post-inner-condition scalar populations of node m
 // (src/passive_scalar/lattice.rs:274-297, node.rs:281-294).

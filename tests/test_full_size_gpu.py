@@ -1,4 +1,5 @@
-"""GPU FP64 parity path against the CPU oracle at the workloads' real sizes.
+"""GPU FP64 paths (two-array and in-place schemes) against the CPU oracle at
+the workloads' real sizes.
 
 - D2Q9 BGK lid-driven cavity, 128x128, tau 0.6, lid 0.1: the whole run of
   20,001 steps (the reference's `--max-iterations 20000` loop runs steps
@@ -29,6 +30,9 @@ def rel_err(a, b):
     return np.abs(a - b).max() / scale
 
 
+SCHEMES = [abi.STREAM_TWO_ARRAY, abi.STREAM_AA]
+
+
 def run_both(sp, steps):
     g = Case(sp)
     o = oracle.OracleCase(sp)
@@ -55,9 +59,10 @@ def check_fields(g, o, sp, what):
     assert g.reduce(abi.REDUCE_NONFINITE) == 0
 
 
-def test_cavity_128_whole_run():
-    sp = configs.cavity(n=128, tau=0.6, lid=0.1)
-    print("[full] cavity 128^2, 20001 steps", flush=True)
+@pytest.mark.parametrize("streaming", SCHEMES)
+def test_cavity_128_whole_run(streaming):
+    sp = configs.cavity(n=128, tau=0.6, lid=0.1).replace(streaming=streaming)
+    print(f"[full] cavity 128^2, 20001 steps, streaming={streaming}", flush=True)
     g, o = run_both(sp, 20001)
     check_fields(g, o, sp, "cavity")
     _, u = o.fields()
@@ -66,9 +71,10 @@ def test_cavity_128_whole_run():
     o.close()
 
 
-def test_porous_256_masks_and_steps():
-    sp = configs.porous_channel(n=256, precision=abi.F64)
-    print("[full] porous 256^3: link tables, 12 steps", flush=True)
+@pytest.mark.parametrize("streaming", SCHEMES)
+def test_porous_256_masks_and_steps(streaming):
+    sp = configs.porous_channel(n=256, precision=abi.F64, streaming=streaming)
+    print(f"[full] porous 256^3: link tables, 12 steps, streaming={streaming}", flush=True)
     g = Case(sp)
     o = oracle.OracleCase(sp)
     t1, m1, f1 = g.link_tables()
