@@ -1,4 +1,5 @@
-// fast.hpp — in-place AA-pattern step kernels for the throughput tiers.
+// fast.hpp — in-place AA-pattern step kernels (FP64 in the parity kernel's
+// arithmetic, FP32 and shifted storage in pair form).
 //
 // One population array F[q][ld]. Even steps read the node's own slots
 // F[j][x] (= post-stream f_j), collide, and write f*_j to the swapped own slot

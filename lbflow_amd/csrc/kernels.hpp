@@ -243,8 +243,9 @@ __device__ __forceinline__ uint32_t block_index(const KArgs& a) {
   return blk;
 }
 
-// Plain-form momentum step (storage holds f). Used by the f64 parity path
-// and the plain FP32 path.
+// Plain-form momentum step (storage holds f), two-array scheme: FP64 and
+// FP32 cases the in-place scheme does not take (Shan-Chen, anti-bounce-back,
+// Zou-He edges in FP64), or on request.
 template <class L, class St, int OP>
 __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
   using T = typename St::T;

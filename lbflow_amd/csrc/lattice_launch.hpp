@@ -81,7 +81,7 @@ static void serve_launchers(Launchers* out) {
 template <class L>
 static bool make_launchers(int precision, Launchers* out) {
   switch (precision) {
-    case 0:  // LBF_F64: parity path (two arrays) and AA on request
+    case 0:  // LBF_F64: parity path, two arrays or in place (same bits)
       out->mom_step = mom_step_plain<L, StF64>;
       out->mom_init = mom_init_t<L, StF64>;
       out->sca_step = sca_step_t<L, double>;
