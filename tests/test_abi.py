@@ -77,3 +77,32 @@ def test_validation_errors_without_gpu():
     desc, keep = two.to_c()
     assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == 0
     assert out.value >= 2 * 19 * 16 ** 3 * 4
+
+
+def test_automatic_scheme_choice_without_gpu():
+    # lbf_device_bytes sizes the scheme lbf_create would pick: one population
+    # array in place, two arrays otherwise
+    lib = load_library()
+    out = C.c_uint64()
+
+    def arrays(sp):
+        desc, keep = sp.to_c()
+        assert lib.lbf_device_bytes(C.byref(desc), C.byref(out)) == 0
+        one = sp.num_nodes * {abi.D2Q9: 9, abi.D3Q15: 15, abi.D3Q19: 19, abi.D3Q27: 27}[
+            sp.velocity_set] * {abi.F64: 8, abi.F32: 4}[sp.precision]
+        return 2 if out.value >= 2 * one else 1
+
+    # FP64 runs in place where that gives the reference's bits ...
+    assert arrays(configs.porous_channel(n=16, precision=abi.F64)) == 1
+    assert arrays(configs.sphere_channel(nx=24, ny=12, nz=12, radius=3)) == 1  # Zou-He x, periodic y/z
+    # ... and on two arrays where a Zou-He face meets another non-periodic face
+    from lbflow_amd.case import FaceBC, Spec
+    channel = Spec(velocity_set=abi.D3Q19, n=(20, 10, 8), collision=configs.Collision.trt(1.2, 0.9),
+                   faces=[FaceBC.zou_he(None, (0.03, 0.0, 0.0)), FaceBC.zou_he(1.0, (None, 0.0, 0.0))]
+                   + [FaceBC.no_slip()] * 4, initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0),
+                   zou_he_3d=True)
+    assert arrays(channel) == 2
+    assert arrays(channel.replace(precision=abi.F32)) == 1  # FP32 is not held to the bits
+    # an in-place array of 4 GiB or more falls back to two arrays
+    assert arrays(configs.porous_channel(n=600, precision=abi.F32, p_solid=0.0)) == 2
+    assert arrays(configs.porous_channel(n=350, precision=abi.F32, p_solid=0.0)) == 1  # 3.3 GB
