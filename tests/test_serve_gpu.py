@@ -188,3 +188,37 @@ def test_zero_steps_returns_the_initial_fields():
         rho, u, _ = srv.result(t)
     fluid = ~spec.node_type.astype(bool).reshape(-1)
     assert np.allclose(rho[fluid], 1.0) and np.allclose(u[fluid], 0.0)
+
+
+def test_clients_on_several_threads_get_their_own_results():
+    # four client threads submit, wait and fetch at the same time; every
+    # result must be bitwise the single-case path's
+    import threading
+    specs = [case(abi.D2Q9 if i % 2 else abi.D3Q19, 30 if i % 2 else 14,
+                  (abi.F32, abi.F16_SHIFTED)[(i // 2) % 2], seed=40 + i) for i in range(8)]
+    steps = [9 + 3 * i for i in range(8)]
+    refs = [single(sp, n) for sp, n in zip(specs, steps)]
+    got = [None] * 8
+    errors = []
+    print("[serve-threads] 4 clients x 2 cases", flush=True)
+    with Server(max_steps_per_round=4) as srv:
+        def client(k):
+            try:
+                mine = [2 * k, 2 * k + 1]
+                tickets = [srv.submit(specs[i], steps[i], keep_pops=True) for i in mine]
+                for i, t in zip(mine, tickets):
+                    srv.wait(t, 120)
+                    got[i] = srv.result(t)
+            except Exception as e:  # reported below
+                errors.append(repr(e))
+        threads = [threading.Thread(target=client, args=(k,)) for k in range(4)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join(180)
+        st = srv.stats()
+    assert not errors, errors
+    assert st["completed"] == 8 and st["failed"] == 0
+    for ref, g in zip(refs, got):
+        for a, b in zip(ref, g):
+            assert np.array_equal(a, b)
