@@ -25,6 +25,10 @@
 // Face conditions are then applied in face order from those values
 // (src/momentum/bc.rs:29-129). This requires opposite faces to be both
 // periodic or both not, which the host checks before choosing this scheme.
+// At an edge where a Zou-He face meets another non-periodic face the closure
+// therefore sees the other face's bounce-back values, where the reference,
+// applying faces one after another, can still hold wrapped pull values; the
+// host's automatic choice keeps such FP64 cases on the two-array scheme.
 //
 // Addressing: one buffer resource over the whole population array, a 32-bit
 // per-thread byte offset (shared by all planes of a neighbour) and a
