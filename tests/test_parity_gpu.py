@@ -44,6 +44,21 @@ def cases():
                                      ScalarFaceBC(abi.SBC_PERIODIC), ScalarFaceBC(abi.SBC_PERIODIC)],
                               inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.2,
                               adsorption=dict(adsorption_rate=0.2, q_max=1.0))]
+    # scalar MRT (src/passive_scalar/node.rs via kernel::mrt_collision): D3Q19 with
+    # bounce-back walls and adsorption, D2Q9 with an anti-bounce-back inlet
+    sc_mrt_3d = configs.porous_channel(n=(12, 10, 8), force=1e-4, p_solid=0.15, seed=9)
+    sc_mrt_3d.scalars = [ScalarSpec(
+        collision=Collision.mrt([1.0, 1.19, 1.4, 1.0, 1.2, 1.0, 1.2, 1.0, 1.2, 1.25, 1.4, 1.25,
+                                 1.4, 1.25, 1.25, 1.25, 1.98, 1.98, 1.98]),
+        faces=[ScalarFaceBC(abi.SBC_PERIODIC)] * 2 + [ScalarFaceBC(abi.SBC_BB_NO_FLUX)] * 4,
+        inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.5,
+        adsorption=dict(adsorption_rate=0.05, q_max=1.0))]
+    sc_mrt_2d = configs.cavity(n=14, tau=0.7)
+    sc_mrt_2d.scalars = [ScalarSpec(
+        collision=Collision.mrt([1.0, 1.4, 1.2, 1.0, 1.1, 1.0, 1.1, 1.3, 1.3]),
+        faces=[ScalarFaceBC(abi.SBC_ANTI_BOUNCE_BACK, 1.0), ScalarFaceBC(abi.SBC_ZEROTH_ORDER_NO_FLUX),
+               ScalarFaceBC(abi.SBC_BB_NO_FLUX), ScalarFaceBC(abi.SBC_BB_NO_FLUX)],
+        inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.0)]
     rng = np.random.default_rng(8)
     sc_mask = np.zeros((20, 16), np.uint8)
     sc_mask[6:10, 5:9] = 1
@@ -81,6 +96,8 @@ def cases():
         "coupled_discs": (configs.coupled_discs(nx=48, ny=24, count=6, rmin=1.5, rmax=3, seed=1, precision=abi.F64, k_a=0.05), 40),
         "coupled_discs_unknown_only": (configs.coupled_discs(nx=40, ny=20, count=5, rmin=1.5, rmax=3, seed=2, precision=abi.F64, k_a=0.05, zeroth_unknown_only=True), 30),
         "two_scalars_mixed_bcs": (two, 25),
+        "scalar_mrt_d3q19_adsorption": (sc_mrt_3d, 20),
+        "scalar_mrt_d2q9_inlet": (sc_mrt_2d, 30),
         "anti_bounce_back_walls": (anti, 20),
         "shan_chen_d2q9_bgk_force_walls": (shan_chen_2d, 40),
         "shan_chen_d3q19_trt_no_slip": (shan_chen_3d, 16),
