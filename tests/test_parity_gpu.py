@@ -59,6 +59,20 @@ def cases():
         faces=[ScalarFaceBC(abi.SBC_ANTI_BOUNCE_BACK, 1.0), ScalarFaceBC(abi.SBC_ZEROTH_ORDER_NO_FLUX),
                ScalarFaceBC(abi.SBC_BB_NO_FLUX), ScalarFaceBC(abi.SBC_BB_NO_FLUX)],
         inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.0)]
+    sc_d3q15 = configs.porous_channel(n=(10, 9, 8), velocity_set=abi.D3Q15, force=1e-4,
+                                      p_solid=0.1, seed=11)
+    sc_d3q15.scalars = [ScalarSpec(
+        collision=Collision.trt(1.3, 0.9),
+        faces=[ScalarFaceBC(abi.SBC_PERIODIC)] * 2 + [ScalarFaceBC(abi.SBC_ANTI_BOUNCE_BACK, 0.8)]
+        + [ScalarFaceBC(abi.SBC_BB_NO_FLUX)] * 3,
+        inner_bc=abi.INNER_ANTI_BOUNCE_BACK, initial_value=0.1, source=2e-3)]
+    sc_d3q27 = configs.porous_channel(n=(9, 8, 7), velocity_set=abi.D3Q27, op=Collision.bgk(0.8),
+                                      force=1e-4, p_solid=0.1, seed=12)
+    sc_d3q27.scalars = [ScalarSpec(
+        collision=Collision.bgk(0.9),
+        faces=[ScalarFaceBC(abi.SBC_PERIODIC)] * 2 + [ScalarFaceBC(abi.SBC_BB_NO_FLUX)] * 4,
+        inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.4,
+        adsorption=dict(adsorption_rate=0.1, desorption_rate=0.02, q_max=0.8, initial_q_ads=0.1))]
     rng = np.random.default_rng(8)
     sc_mask = np.zeros((20, 16), np.uint8)
     sc_mask[6:10, 5:9] = 1
@@ -98,6 +112,8 @@ def cases():
         "two_scalars_mixed_bcs": (two, 25),
         "scalar_mrt_d3q19_adsorption": (sc_mrt_3d, 20),
         "scalar_mrt_d2q9_inlet": (sc_mrt_2d, 30),
+        "scalar_trt_d3q15_source": (sc_d3q15, 20),
+        "scalar_bgk_d3q27_adsorption": (sc_d3q27, 16),
         "anti_bounce_back_walls": (anti, 20),
         "shan_chen_d2q9_bgk_force_walls": (shan_chen_2d, 40),
         "shan_chen_d3q19_trt_no_slip": (shan_chen_3d, 16),
