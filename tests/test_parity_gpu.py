@@ -73,6 +73,20 @@ def cases():
         faces=[ScalarFaceBC(abi.SBC_PERIODIC)] * 2 + [ScalarFaceBC(abi.SBC_BB_NO_FLUX)] * 4,
         inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.4,
         adsorption=dict(adsorption_rate=0.1, desorption_rate=0.02, q_max=0.8, initial_q_ads=0.1))]
+    # per-node force and scalar source fields (the reference's closures, evaluated per node)
+    frng = np.random.default_rng(21)
+    ff_3d = configs.porous_channel(n=(10, 9, 8), p_solid=0.2, seed=13).replace(
+        force=1e-4 * frng.standard_normal((720, 3)))
+    ff_2d = Spec(velocity_set=abi.D2Q9, n=(16, 12), collision=Collision.bgk(0.75),
+                 faces=[FaceBC.periodic()] * 2 + [FaceBC.no_slip()] * 2,
+                 node_type=geometry.bernoulli_mask((16, 12), 0.15, 14),
+                 initial_density=1.0, initial_velocity=(0.0, 0.0),
+                 force=1e-4 * frng.standard_normal((192, 2)))
+    ff_2d.scalars = [ScalarSpec(
+        collision=Collision.bgk(0.8),
+        faces=[ScalarFaceBC(abi.SBC_PERIODIC)] * 2 + [ScalarFaceBC(abi.SBC_BB_NO_FLUX)] * 2,
+        inner_bc=abi.INNER_BOUNCE_BACK, initial_value=0.3,
+        source=1e-3 * frng.standard_normal(192))]
     rng = np.random.default_rng(8)
     sc_mask = np.zeros((20, 16), np.uint8)
     sc_mask[6:10, 5:9] = 1
@@ -114,6 +128,8 @@ def cases():
         "scalar_mrt_d2q9_inlet": (sc_mrt_2d, 30),
         "scalar_trt_d3q15_source": (sc_d3q15, 20),
         "scalar_bgk_d3q27_adsorption": (sc_d3q27, 16),
+        "force_field_d3q19_trt": (ff_3d, 18),
+        "force_field_d2q9_bgk_scalar_source_field": (ff_2d, 24),
         "anti_bounce_back_walls": (anti, 20),
         "shan_chen_d2q9_bgk_force_walls": (shan_chen_2d, 40),
         "shan_chen_d3q19_trt_no_slip": (shan_chen_3d, 16),
