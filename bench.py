@@ -6,8 +6,9 @@ along x (magic parameter 3/16, tau 0.8), populations in FP32. One "step" is
 one lattice time step of the whole 256^3 lattice. MLUPS = N * steps / t / 1e6.
 
 Prints one JSON line. Besides the headline (FP32), the same workload is
-timed in f64 (the parity path) and in the shifted FP16 storage tier when it
-is built, and reported under "extra". The roofline block uses algorithmic
+timed under "extra" in FP64 (f64_aa: the default FP64 path, in place, with
+the reference's bits; f64: the two-array scheme), FP32 on two arrays and the
+shifted FP32/FP16/BF16 storage tiers. The roofline block uses algorithmic
 bytes 2*Q*s per node update; the CPU baseline times the oracle (the
 line-by-line f64 restatement of the reference solver) on the host cores for
 a bounded sample of the same workload. The same kernel on an all-fluid, fully
