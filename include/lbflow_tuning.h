@@ -46,6 +46,14 @@ int lbfx_set_block_group(lbf_case* c, int g);
  * Returns -1 for an invalid pad. */
 int lbfx_set_plane_pad(long long elements);
 
+/* The engine's MRT matrices of velocity set vs (LBF_D2Q9 .. LBF_D3Q27) as the
+ * kernels use them: M[k*Q + i] (moment k of direction i) and
+ * Minv[i*Q + k]. D2Q9 / D3Q15 / D3Q19 follow the reference's tables
+ * (d2q9.rs:44-156, d3q15.rs:59-363, d3q19.rs:69-548); D3Q27 is this engine's
+ * own tensor-product basis (the reference has none, d3q27.rs:86). Host only,
+ * no device call. Returns -1 for a set without MRT. */
+int lbfx_mrt_matrices(int velocity_set, double* M, double* Minv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,6 +158,7 @@ TUNING_ENTRY_POINTS = [
     ("lbfx_case_stream", _vp, [_vp]),
     ("lbfx_set_block_group", C.c_int, [_vp, C.c_int]),
     ("lbfx_set_plane_pad", C.c_int, [C.c_longlong]),
+    ("lbfx_mrt_matrices", C.c_int, [C.c_int, _dp, _dp]),
 ]
 
 SERVER_ENTRY_POINTS = [

@@ -23,6 +23,7 @@
 #include "dispatch.hpp"
 #include "internal.hpp"
 #include "engine.hpp"
+#include "lattice.hpp"
 
 using namespace lbf;
 
@@ -1084,6 +1085,30 @@ extern "C" int lbfx_set_block_group(lbf_case* c, int g) {
   if (!c || g < 0 || (g > 0 && c->grid % (8 * g) != 0)) return -1;
   c->xcd_group = g;
   return 0;
+}
+
+namespace {
+template <class L>
+void mrt_tables(double* M, double* Minv) {
+  for (int k = 0; k < L::Q; k++)
+    for (int i = 0; i < L::Q; i++) {
+      M[k * L::Q + i] = Derived<L>::M(k, i);
+      Minv[i * L::Q + k] = Derived<L>::Minv(i, k);
+    }
+}
+}  // namespace
+
+// Inspection hook (not part of the reference-equivalent ABI): the MRT
+// matrices the kernels are built with (constexpr tables of lattice.hpp).
+extern "C" int lbfx_mrt_matrices(int vs, double* M, double* Minv) {
+  if (!M || !Minv) return -1;
+  switch (vs) {
+    case LBF_D2Q9: mrt_tables<D2Q9>(M, Minv); return 0;
+    case LBF_D3Q15: mrt_tables<D3Q15>(M, Minv); return 0;
+    case LBF_D3Q19: mrt_tables<D3Q19>(M, Minv); return 0;
+    case LBF_D3Q27: mrt_tables<D3Q27>(M, Minv); return 0;
+  }
+  return -1;
 }
 
 // Tuning hook (not part of the reference-equivalent ABI): pad, in elements (a

@@ -159,6 +159,24 @@ static double mrt_row(int id, int k, const int* cv) {
       case 17: return (z * z - x * x) * y;
       case 18: return (x * x - y * y) * z;
     }
+  } else if (id == LBF_D3Q27) {
+    /* The reference has no D3Q27 MRT basis (MRT_MATRIX is an all-ones
+     * placeholder, d3q27.rs:86, and the inverse is todo!(),
+     * velocity_set.rs:168). This is the engine's own basis: rows are the
+     * tensor products p_a(cx) p_b(cy) p_c(cz) of p0 = 1, p1 = a, p2 = 3a^2 - 2
+     * (orthogonal over a in {-1, 0, 1}), (a, b, c) in this order. */
+    static const int ORD27[27][3] = {
+        {0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2},
+        {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {2, 1, 0}, {2, 0, 1}, {1, 2, 0}, {0, 2, 1},
+        {1, 0, 2}, {0, 1, 2}, {1, 1, 1}, {2, 2, 0}, {2, 0, 2}, {0, 2, 2}, {2, 1, 1},
+        {1, 2, 1}, {1, 1, 2}, {2, 2, 1}, {2, 1, 2}, {1, 2, 2}, {2, 2, 2}};
+    const double a[3] = {x, y, z};
+    double r = 1.0;
+    for (int ax = 0; ax < 3; ax++) {
+      const int o = ORD27[k][ax];
+      r *= o == 0 ? 1.0 : (o == 1 ? a[ax] : 3.0 * a[ax] * a[ax] - 2.0);
+    }
+    return r;
   }
   return 0;
 }
@@ -209,8 +227,11 @@ static int vset_init(int id, vset* v) {
       for (int f = 0; f < 6; f++)
         for (int j = 0; j < 9; j++) v->qfaces[f][j] = QF_D3Q27[f][j];
       /* MRT_MATRIX is a placeholder and the inverse is todo!() (d3q27.rs:86,
-       * velocity_set.rs:168): MRT is not constructible in the reference. */
-      v->has_mrt = 0;
+       * velocity_set.rs:168): MRT is not constructible in the reference; it
+       * runs here in the engine's own basis (mrt_row). PARITY UNPINNED by the
+       * reference: pinned by mathematics in tests/test_oracle.py (M M^-1 = I,
+       * uniform rates equal BGK, conservation, one relaxed mode at a time). */
+      v->has_mrt = 1;
       break;
     default:
       return -1;
@@ -331,8 +352,11 @@ static void k_meq(int id, double rho, const double* u, double* m) {
   }
 }
 
-/* src/kernel.rs:67-120 (m_eq always from the closed form: every
- * constructible set provides one, velocity_set.rs:79-81,110-112,141-143) */
+/* src/kernel.rs:67-120: m_eq from the closed form where the set has one
+ * (D2Q9, D3Q15, D3Q19: velocity_set.rs:79-81,110-112,141-143), otherwise
+ * m_eq = M f_eq (kernel.rs:92-105, the `None` branch; D3Q27 with the
+ * engine's basis). f_eq is recomputed from (value, u), which gives the bits
+ * of the node's stored f_eq (the same function of the same inputs). */
 static void k_mrt(const vset* v, double value, const double* u, const double* f,
                   const double* s, double* fs) {
   double m[27], meq[27] = {0}, ms[27];
@@ -342,7 +366,17 @@ static void k_mrt(const vset* v, double value, const double* u, const double* f,
     for (int i = 1; i < q; i++) acc = acc + v->M[k][i] * f[i];
     m[k] = acc;
   }
-  k_meq(v->id, value, u, meq);
+  if (v->id == LBF_D3Q27) {
+    double feq[27];
+    k_equilibrium(v, value, u, feq);
+    for (int k = 0; k < q; k++) {
+      double acc = v->M[k][0] * feq[0];
+      for (int i = 1; i < q; i++) acc = acc + v->M[k][i] * feq[i];
+      meq[k] = acc;
+    }
+  } else {
+    k_meq(v->id, value, u, meq);
+  }
   for (int k = 0; k < q; k++) ms[k] = m[k] - s[k] * DELTA_T * (m[k] - meq[k]);
   for (int i = 0; i < q; i++) {
     double acc = v->Minv[i][0] * ms[0];

@@ -12,6 +12,12 @@ Parity status of each oracle function (what pins it):
   TRT                     TRT(w, w) == BGK(1/w) + conservation + fixed point
   MRT                     conservation; uniform rates == BGK for D3Q15; the
                           D2Q9/D3Q19 closed-form m_eq quirks detected
+  D3Q27 MRT (own basis;   no reference (d3q27.rs:86 placeholder,
+   not in the reference)  velocity_set.rs:168 todo!()): M rebuilt here from the
+                          tensor polynomials, orthogonality, M M^-1 = I,
+                          uniform rates == BGK (m_eq = M f_eq), conservation
+                          for any rates, one relaxed mode moves only that
+                          moment; the library's tables equal the oracle's
   Guo source              sum S = 0, sum S c = (1 - 1/2tau) F
   scalar source           sum S = (1 - 1/2tau_g) S
   Zou-He                  the node reproduces the prescribed rho / u
@@ -105,10 +111,65 @@ def test_mrt_rows_match_reference_text():
     assert Mi[0][0] == 2.0 / 30.0 and Mi[7][2] == 1.0 / 360.0 and Mi[0][2] == 4.0 / 90.0
 
 
-@pytest.mark.parametrize("vs", [abi.D2Q9, abi.D3Q15, abi.D3Q19])
+@pytest.mark.parametrize("vs", [abi.D2Q9, abi.D3Q15, abi.D3Q19, abi.D3Q27])
 def test_mrt_inverse(vs):
     M, Mi = oracle.mrt_matrices(vs)
     assert np.abs(M @ Mi - np.eye(len(M))).max() < 1e-15
+
+
+# D3Q27 MRT basis of this engine: rows p_a(cx) p_b(cy) p_c(cz) with p0 = 1,
+# p1 = a, p2 = 3a^2 - 2, taken in this (total order, then lexicographic) order
+D3Q27_ORDERS = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (2, 0, 0), (0, 2, 0), (0, 0, 2),
+                (1, 1, 0), (1, 0, 1), (0, 1, 1), (2, 1, 0), (2, 0, 1), (1, 2, 0), (0, 2, 1),
+                (1, 0, 2), (0, 1, 2), (1, 1, 1), (2, 2, 0), (2, 0, 2), (0, 2, 2), (2, 1, 1),
+                (1, 2, 1), (1, 1, 2), (2, 2, 1), (2, 1, 2), (1, 2, 2), (2, 2, 2)]
+
+
+def test_d3q27_mrt_basis():
+    polys = [np.poly1d([1]), np.poly1d([1, 0]), np.poly1d([3, 0, -2])]
+    c = oracle.tables(abi.D3Q27)["c"]
+    M_ref = np.array([[polys[a](ci[0]) * polys[b](ci[1]) * polys[g](ci[2]) for ci in c]
+                      for a, b, g in D3Q27_ORDERS])
+    assert sorted(D3Q27_ORDERS) == sorted({(a, b, g) for a in range(3) for b in range(3)
+                                           for g in range(3)})
+    M, Mi = oracle.mrt_matrices(abi.D3Q27)
+    assert np.array_equal(M, M_ref)
+    # density and momentum rows; orthogonal rows; M^-1 = M^T diag(1/|row|^2)
+    assert np.array_equal(M[0], np.ones(27))
+    assert np.array_equal(M[1:4], c.T)
+    G = M @ M.T
+    assert np.array_equal(G, np.diag(np.diag(G)))
+    assert np.array_equal(Mi, M.T / np.diag(G))
+
+
+def test_d3q27_mrt_one_relaxed_mode():
+    # with only s_k non-zero, M (f* - f) = -s_k (m_k - meq_k) e_k: a transposed
+    # or reordered M / M^-1 moves other moments too
+    M, _ = oracle.mrt_matrices(abi.D3Q27)
+    rng = np.random.default_rng(27)
+    rho, u = 1.02, np.array([0.03, -0.02, 0.04])
+    feq = oracle.k_equilibrium(abi.D3Q27, rho, u)
+    f = feq * (1 + rng.uniform(-0.05, 0.05, 27))
+    for k in (0, 1, 4, 7, 16, 20, 26):
+        s = np.zeros(27)
+        s[k] = 1.3
+        d = M @ (oracle.k_mrt(abi.D3Q27, rho, u, f, s) - f)
+        want = np.zeros(27)
+        want[k] = -1.3 * (M[k] @ f - M[k] @ feq)
+        assert np.abs(d - want).max() < 1e-14, k
+
+
+@pytest.mark.parametrize("vs", [abi.D2Q9, abi.D3Q15, abi.D3Q19, abi.D3Q27])
+def test_library_mrt_tables_equal_oracle(vs):
+    # the GPU library's constexpr tables (lattice.hpp), read on the host
+    import ctypes as C
+    from lbflow_amd.engine import load_library
+    q = abi.Q_OF[vs]
+    M, Mi = np.zeros(q * q), np.zeros(q * q)
+    dp = C.POINTER(C.c_double)
+    assert load_library().lbfx_mrt_matrices(vs, M.ctypes.data_as(dp), Mi.ctypes.data_as(dp)) == 0
+    Mo, Mio = oracle.mrt_matrices(vs)
+    assert np.array_equal(M.reshape(q, q), Mo) and np.array_equal(Mi.reshape(q, q), Mio)
 
 
 # --------------------------------------------------------- per-node kernels
@@ -142,7 +203,7 @@ def _moments(vs, f):
     return f.sum(), f @ c
 
 
-@pytest.mark.parametrize("vs", [abi.D2Q9, abi.D3Q15, abi.D3Q19])
+@pytest.mark.parametrize("vs", [abi.D2Q9, abi.D3Q15, abi.D3Q19, abi.D3Q27])
 def test_collisions_conserve_and_relate(vs):
     rng = np.random.default_rng(1)
     q = abi.Q_OF[vs]
@@ -164,8 +225,9 @@ def test_collisions_conserve_and_relate(vs):
         m = oracle.k_mrt(vs, r0, j0 / r0, f, s)
         r, j = _moments(vs, m)
         assert abs(r - r0) < 1e-13 and np.abs(j - j0).max() < 1e-14
-        if vs == abi.D3Q15:
-            # closed-form m_eq equals M f_eq for D3Q15: uniform rates give BGK
+        if vs in (abi.D3Q15, abi.D3Q27):
+            # m_eq equals M f_eq (D3Q15's closed form; D3Q27 computes it so,
+            # kernel.rs:92-105): uniform rates give BGK
             assert np.abs(m - b).max() < 1e-14
         else:
             # D2Q9 / D3Q19 closed forms deviate from M f_eq (d2q9.rs:247-264,

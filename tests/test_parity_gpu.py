@@ -137,6 +137,13 @@ def cases():
         "zou_he_3d_d3q15_mixed_faces": (zh_mixed, 20),
         "zou_he_3d_d3q27_sphere": (configs.sphere_channel(nx=24, ny=12, nz=12, radius=3,
                                                           velocity_set=abi.D3Q27), 20),
+        # D3Q27 MRT in the engine's basis (the reference has none; the oracle's
+        # restatement of it is pinned by mathematics in tests/test_oracle.py)
+        "zou_he_3d_d3q27_mrt_sphere": (configs.sphere_channel(
+            nx=24, ny=12, nz=12, radius=3, op=Collision.mrt([1 / 0.6] * 10 + [1.2] * 17)), 16),
+        "d3q27_mrt_porous": (configs.porous_channel(
+            n=(9, 8, 7), velocity_set=abi.D3Q27, force=1e-4,
+            op=Collision.mrt([1.0] * 4 + [1.3, 1.25, 1.2, 1.15, 1.1, 1.05] + [1.4 - 0.02 * k for k in range(17)])), 12),
     }
 
 

@@ -39,7 +39,16 @@ AA_CASES = {
     "cavity_mrt_d2q9": configs.cavity(n=20, op=Collision.mrt([1.0, 1.4, 1.2, 1.0, 1.1, 1.0, 1.1, 1.3, 1.3])),
     "zou_he_3d_sphere_d3q19": configs.sphere_channel(nx=24, ny=12, nz=10, radius=3,
                                                      velocity_set=abi.D3Q19, op=Collision.trt(1.1, 0.9)),
-    "zou_he_3d_sphere_d3q27": configs.sphere_channel(nx=20, ny=10, nz=12, radius=3),
+    "zou_he_3d_sphere_d3q27_bgk": configs.sphere_channel(nx=20, ny=10, nz=12, radius=3),
+    # D3Q27 MRT in the engine's own basis (no reference counterpart): the
+    # benchmark's rates on the sphere, and distinct rates on a porous channel;
+    # FP64 runs the dense M / M^-1 products, FP32 and shifted storage the
+    # separable tensor-product transforms
+    "zou_he_3d_sphere_d3q27_mrt": configs.sphere_channel(
+        nx=20, ny=10, nz=12, radius=3, op=Collision.mrt([1 / 0.6] * 10 + [1.2] * 17)),
+    "porous_mrt_d3q27": configs.porous_channel(
+        n=(10, 8, 9), velocity_set=abi.D3Q27, force=1e-4,
+        op=Collision.mrt([1.0] * 4 + [1.3, 1.25, 1.2, 1.15, 1.1, 1.05] + [1.4 - 0.02 * k for k in range(17)])),
 }
 
 
@@ -151,19 +160,70 @@ def test_aa_set_pops_odd_and_even():
 
 
 @pytest.mark.parametrize("steps", [7, 8])
-def test_aa_d3q27_mrt_matches_two_array_path(steps):
-    # the reference has no D3Q27 MRT (and the oracle refuses it); in FP64 the
-    # in-place kernel runs the two-array kernel's arithmetic (dense M / M^-1
-    # products of the same basis), so the two schemes give the same bits
+def test_aa_d3q27_mrt_matches_two_array_path_and_oracle(steps):
+    # the reference has no D3Q27 MRT; the oracle runs the engine's basis
+    # (pinned by mathematics in tests/test_oracle.py). In FP64 both schemes run
+    # the dense products in the oracle's order: the same bits.
     sp = configs.porous_channel(n=(10, 8, 9), velocity_set=abi.D3Q27, force=1e-4,
                                 op=Collision.mrt([1.0] * 4 + [1.3] * 6 + [1.1] * 17))
     print(f"[aa-d3q27-mrt] steps={steps}", flush=True)
     g = Case(sp.replace(precision=abi.F64, streaming=abi.STREAM_AA))
     t = Case(sp.replace(precision=abi.F64, streaming=abi.STREAM_TWO_ARRAY))
-    for c in (g, t):
+    o = oracle.OracleCase(sp.replace(precision=abi.F64))
+    for c in (g, t, o):
         c.initialize_nodes()
         c.main_steps(steps)
-    assert np.array_equal(g.pops(), t.pops())
+    assert np.array_equal(g.pops(), t.pops()) and np.array_equal(g.pops(), o.pops())
     rg, ug = g.fields()
     rt, ut = t.fields()
+    ro, uo = o.fields()
     assert np.array_equal(rg, rt) and np.array_equal(ug, ut)
+    assert np.array_equal(rg, ro) and np.array_equal(ug, uo)
+
+
+# measured on MI355X: FP64 3.0e-15, FP32 9.8e-7, FP16-shifted 4.9e-5 (the two
+# operators round differently before each 16-bit store)
+@pytest.mark.parametrize("prec,tol", [(abi.F64, 1e-13), (abi.F32, 5e-6), (abi.F16_SHIFTED, 3e-4)])
+def test_d3q27_mrt_uniform_rates_equal_bgk(prec, tol):
+    # m_eq = M f_eq, so MRT with every rate 1/tau is BGK(tau): f* = f - (f - f_eq)/tau.
+    # FP64 checks the dense products, FP32 / FP16-shifted the tensor-product
+    # transforms (a wrong stride, order or inverse weight breaks the identity
+    # at O(1)); no force, since Guo forcing applies to BGK only
+    tau = 0.7
+    base = configs.sphere_channel(nx=24, ny=12, nz=12, radius=3, tau=tau, precision=prec)
+    out = []
+    for op in (Collision.bgk(tau), Collision.mrt([1 / tau] * 27)):
+        g = Case(base.replace(collision=op, streaming=abi.STREAM_AA))
+        g.initialize_nodes()
+        g.main_steps(15)
+        out.append(g.pops())
+        g.close()
+    e = rel(out[1], out[0])
+    print(f"[d3q27-mrt=bgk] prec={prec} max rel diff {e:.2e}", flush=True)
+    assert e < tol
+
+
+def test_coupled_discs_small_f32_against_oracle():
+    # FP32 coupled momentum + scalar with Zou-He inlet/outlet, anti-bounce-back
+    # scalar inlet and Langmuir adsorption on the discs (two-array kernels
+    # momentum_step_plain<D2Q9, FP32> and scalar_step<D2Q9, float>), 200 steps
+    sp = configs.coupled_discs(nx=48, ny=24, count=6, rmin=1.5, rmax=3, seed=1, k_a=0.05)
+    assert sp.precision == abi.F32
+    g = Case(sp)
+    o = oracle.OracleCase(sp.replace(precision=abi.F64))
+    g.initialize_nodes()
+    o.initialize_nodes()
+    g.main_steps(200)
+    o.main_steps(200)
+    fluid = sp.node_type == 0
+    _, ug = g.fields()
+    _, uo = o.fields()
+    cg, qg = g.scalar(0)
+    co, qo = o.scalar(0)
+    e_u = l2rel(ug[fluid], uo[fluid])
+    e_c = l2rel(cg[fluid], co[fluid])
+    qi_o = o.reduce(abi.REDUCE_ADSORBED, 0)
+    e_q = abs(g.reduce(abi.REDUCE_ADSORBED, 0) - qi_o) / qi_o
+    print(f"[coupled-f32] L2 u {e_u:.2e}, L2 c {e_c:.2e}, sum q_ads {e_q:.2e}", flush=True)
+    assert g.reduce(abi.REDUCE_NONFINITE) == 0 and qi_o > 0
+    assert e_u < 1e-4 and e_c < 1e-4 and e_q < 1e-4
