@@ -246,6 +246,15 @@ int lbf_get_scalar(lbf_case* c, uint32_t s, double* value, double* q_ads);
 /* Post-stream populations (the `f` / `g` vectors of the reference nodes) in
  * reference AoS order out[node*Q + i]. field 0 = momentum, 1+s = scalar s. */
 int lbf_get_pops(lbf_case* c, int32_t field, double* out);
+/* lbf_get_pops without any device or host allocation. No counterpart in the
+ * reference: this is for callers that cannot allocate (code in a tight
+ * serving or time loop, harnesses that forbid hipMalloc). The populations
+ * pass through a fixed 16 MiB device array of the library (one per GPU, kept
+ * until the process ends) in chunks of at most 16 MiB / (8 Q) nodes, so every
+ * lattice size is accepted and no request needs more than that array. Calls
+ * are serialised inside the process (they share the array); the result is
+ * the same as lbf_get_pops. */
+int lbf_get_pops_noalloc(lbf_case* c, int32_t field, double* out);
 /* Overwrite the post-stream populations (parity tests, restarts). */
 int lbf_set_pops(lbf_case* c, int32_t field, const double* in);
 

@@ -113,6 +113,7 @@ ENTRY_POINTS = [
     ("lbf_get_fields", C.c_int, [_vp, _dp, _dp]),
     ("lbf_get_scalar", C.c_int, [_vp, C.c_uint32, _dp, _dp]),
     ("lbf_get_pops", C.c_int, [_vp, C.c_int32, _dp]),
+    ("lbf_get_pops_noalloc", C.c_int, [_vp, C.c_int32, _dp]),
     ("lbf_set_pops", C.c_int, [_vp, C.c_int32, _dp]),
     ("lbf_get_link_tables", C.c_int, [_vp, _u8p, C.POINTER(C.c_uint32), _u8p]),
     ("lbf_device_bytes", C.c_int, [C.POINTER(CaseDesc), C.POINTER(C.c_uint64)]),

@@ -1018,6 +1018,85 @@ int lbf_get_pops(lbf_case* c, int32_t field, double* out) {
   return 0;
 }
 
+}  // extern "C"
+
+// ---- read-back without allocation (lbf_get_pops_noalloc)
+// Device scratch of the allocation-free read-back: one copy per GPU, held
+// until the process ends. Host code never passes this name to a kernel or a
+// copy (on the host it is only a stand-in for the symbol); it uses the
+// per-device address hipGetSymbolAddress returns.
+static constexpr size_t kNoAllocScratchBytes = 16u << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / 8];
+
+namespace {
+std::mutex g_scratch_mu;          // serialises use of each device's array
+double* g_scratch_ptr[64] = {};   // per-device address of `scratch`
+
+// Address of this device's scratch array, looked up on its first use with
+// the device current. Returns 0 or an error code.
+int scratch_address(int device, double** out) {
+  if (device < 0 || device >= 64) return fail(LBF_EDEVICE, "device %d out of range", device);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != device) HIPCHK(hipSetDevice(device));
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device) (void)hipSetDevice(cur);
+    if (e != hipSuccess || !p)
+      return fail(LBF_EDEVICE, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = (double*)p;
+  }
+  *out = g_scratch_ptr[device];
+  return 0;
+}
+}  // namespace
+
+extern "C" {
+
+int lbf_get_pops_noalloc(lbf_case* c, int32_t field, double* out) {
+  if (!c || !out) return fail(LBF_EINVAL, "NULL argument");
+  if (field < 0 || field > c->ns) return fail(LBF_EINVAL, "no field %d", field);
+  // nodes per chunk: Q f64 values per node must fit the scratch array
+  const long long chunk = (long long)(kNoAllocScratchBytes / (8ull * (unsigned)c->q));
+  if (chunk < 1)
+    return fail(LBF_EINVAL, "%d populations per node exceed the %zu-byte scratch array", c->q,
+                kNoAllocScratchBytes);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  double* dev = nullptr;
+  int rc = scratch_address(c->device, &dev);
+  if (rc) return rc;
+  {
+    Ctrl h;  // synchronises and resynchronises the host step count
+    if ((rc = read_ctrl(c, &h))) return rc;
+  }
+  for (long long n0 = 0; n0 < c->N; n0 += chunk) {
+    const long long n1 = std::min<long long>(c->N, n0 + chunk);
+    KArgs a = field == 0 ? momentum_args(c) : scalar_args(c, field - 1);
+    a.mode = kModeStreamOnly;
+    a.flags = 0;
+    a.xcd_group = 0;
+    a.out = dev;
+    a.out_aos = 1;
+    a.out_n0 = n0;
+    a.out_n1 = n1;
+    // every node of the range writes its Q values (zeros on solid nodes), so
+    // nothing has to be cleared first
+    const int grid = (int)((n1 - n0 + kBlock - 1) / kBlock);
+    if (field == 0 && c->aa)
+      HIPCHK(c->L.aa_readback(a, grid, c->stream));
+    else if (field == 0)
+      HIPCHK(c->L.mom_step(a, grid, c->stream));
+    else
+      HIPCHK(c->L.sca_step(a, grid, c->stream));
+    // the next chunk's launch follows this copy on the same stream
+    HIPCHK(hipMemcpyAsync(out + n0 * c->q, dev, (size_t)(n1 - n0) * c->q * 8,
+                          hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int lbf_set_pops(lbf_case* c, int32_t field, const double* in) {
   if (!c || !in) return fail(LBF_EINVAL, "NULL argument");
   if (field < 0 || field > c->ns) return fail(LBF_EINVAL, "no field %d", field);

@@ -62,7 +62,10 @@ struct KArgs {
   uint32_t plane_bytes;  // AA: bytes per population plane (array < 4 GiB)
   int xcd_group;         // runs of node blocks per XCD (0 = round-robin order)
   void* pop[2];          // ping-pong population buffers (two-array scheme)
-  void* out;             // stream-only destination
+  void* out;             // stream-only destination: f64 [q][ld] over every node, or
+                         // with out_aos f64 [node - out_n0][q] over [out_n0, out_n1)
+  int out_aos;
+  long long out_n0, out_n1;
   const uint32_t* cls;
   void* rho[2];          // momentum moment buffers (compute type)
   void* u[2];            // [d][ld] each

@@ -758,22 +758,27 @@ __global__ void __launch_bounds__(kBlock) momentum_readback_aa(KArgs a) {
   const bool fresh = a.fresh != 0;
   const __amdgpu_buffer_rsrc_t rs =
       __builtin_amdgcn_make_buffer_rsrc(a.pop[0], (short)0, (int)0xffffffff, 0x00020000);
-  double* outd = (double*)a.out;
-  const uint32_t N = (uint32_t)a.N;
-  for (uint32_t n = blockIdx.x * kBlock + threadIdx.x; n < N; n += gridDim.x * kBlock) {
+  long long nbeg, nend;
+  readback_range(a, &nbeg, &nend);
+  for (uint32_t n = (uint32_t)nbeg + blockIdx.x * kBlock + threadIdx.x; n < (uint32_t)nend;
+       n += gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
-    if (cls & kSolidBit) continue;
+    if (cls & kSolidBit) {
+      readback_store<Q, double>(a, n, nullptr);
+      continue;
+    }
     NodePos p;
     node_pos<L>(a, n, true, p);
     T f[Q];
     aa_gather<L, St, ODD>(rs, a.plane_bytes, n, fresh, cls & kLinkMask, face_mask<L>(p.on), p, f);
     if (p.on && !fresh && !a.faces_noslip_only)
       aa_faces<L, T, St::shifted>(a, p.on, T(a.scale), T(a.inv_scale), f);
+    double fd[Q];
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;
-      outd[(size_t)j * a.ld + n] =
-          St::shifted ? (double)f[j] * a.inv_scale + k::w(j) : (double)f[j];
+      fd[j] = St::shifted ? (double)f[j] * a.inv_scale + k::w(j) : (double)f[j];
     });
+    readback_store<Q>(a, n, fd);
   }
 }
 

@@ -243,6 +243,30 @@ __device__ __forceinline__ uint32_t block_index(const KArgs& a) {
   return blk;
 }
 
+// Stream-only read-back of one node's post-stream populations as f64: SoA
+// out[j][ld] over every node, or AoS out[(n - out_n0) * Q + j] (solid nodes
+// written as zeros) when a.out_aos is set; then the launch only visits
+// [out_n0, out_n1) (readback_range).
+template <int Q, class T>
+__device__ __forceinline__ void readback_store(const KArgs& a, long long n, const T* f) {
+  double* __restrict__ outd = (double*)a.out;
+  sfor<Q>([&](auto J) {
+    constexpr int j = decltype(J)::value;
+    if (a.out_aos)
+      outd[(n - a.out_n0) * Q + j] = f ? (double)f[j] : 0.0;
+    else if (f)
+      outd[j * a.ld + n] = (double)f[j];
+  });
+}
+
+// Node range a launch visits: every node, or [out_n0, out_n1) for an AoS
+// read-back.
+__device__ __forceinline__ void readback_range(const KArgs& a, long long* n0, long long* n1) {
+  const bool r = a.mode == kModeStreamOnly && a.out_aos;
+  *n0 = r ? a.out_n0 : 0;
+  *n1 = r ? a.out_n1 : a.N;
+}
+
 // Plain-form momentum step (storage holds f), two-array scheme: FP64 and
 // FP32 cases the in-place scheme does not take (Shan-Chen, anti-bounce-back,
 // Zou-He edges in FP64), or on request.
@@ -260,7 +284,6 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
   T* phi = (T*)a.phi;
   const S* __restrict__ src = (const S*)a.pop[par];
   S* __restrict__ dst = (S*)a.pop[par ^ 1];
-  double* __restrict__ outd = (double*)a.out;
   T* rho_c = (T*)a.rho[par];
   T* u_c = (T*)a.u[par];
   const T* rho_p = (const T*)a.rho[par ^ 1];
@@ -271,10 +294,15 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
           coeff_b = T(a.coeff_b);
   double acc[5] = {0, 0, 0, 0, 0};
   const long long ld = a.ld;
-  for (long long n = (long long)block_index(a) * kBlock + threadIdx.x; n < a.N;
+  long long nbeg, nend;
+  readback_range(a, &nbeg, &nend);
+  for (long long n = nbeg + (long long)block_index(a) * kBlock + threadIdx.x; n < nend;
        n += (long long)gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
-    if (cls & kSolidBit) continue;
+    if (cls & kSolidBit) {
+      if (a.mode == kModeStreamOnly) readback_store<Q, T>(a, n, nullptr);
+      continue;
+    }
     T f[Q];
     if (fresh) {
       sfor<Q>([&](auto J) {
@@ -296,10 +324,7 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
       if (on) apply_momentum_faces<L, St>(a, on, f, src, n, g);
     }
     if (a.mode == kModeStreamOnly) {
-      sfor<Q>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        outd[j * ld + n] = (double)f[j];
-      });
+      readback_store<Q>(a, n, f);
       continue;
     }
     T rho, u[3];
@@ -513,7 +538,6 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
   const bool fresh = ctrl->fresh[a.field] != 0;
   const T* __restrict__ src = (const T*)a.pop[par];
   T* __restrict__ dst = (T*)a.pop[par ^ 1];
-  double* __restrict__ outd = (double*)a.out;
   T* c_c = (T*)a.c[par];
   const T* c_p = (const T*)a.c[par ^ 1];
   const T* u_c = (const T*)a.u[par];
@@ -522,10 +546,15 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
   double acc = 0.0;
   const T omega = T(a.omega), omega_p = T(a.omega_p), wp = T(a.wp), wm = T(a.wm),
           coeff_b = T(a.coeff_b);
-  for (long long n = (long long)block_index(a) * kBlock + threadIdx.x; n < a.N;
+  long long nbeg, nend;
+  readback_range(a, &nbeg, &nend);
+  for (long long n = nbeg + (long long)block_index(a) * kBlock + threadIdx.x; n < nend;
        n += (long long)gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
-    if (cls & kSolidBit) continue;
+    if (cls & kSolidBit) {
+      if (a.mode == kModeStreamOnly) readback_store<Q, T>(a, n, nullptr);
+      continue;
+    }
     T gv[Q];
     Geo g;
     g.init(n, a.nx, a.ny, a.nz);
@@ -627,10 +656,7 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
       }
     }
     if (a.mode == kModeStreamOnly) {
-      sfor<Q>([&](auto J) {
-        constexpr int j = decltype(J)::value;
-        outd[j * ld + n] = (double)gv[j];
-      });
+      readback_store<Q>(a, n, gv);
       continue;
     }
     const bool bb = (cls & kLinkMask) != 0;

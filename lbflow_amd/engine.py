@@ -144,6 +144,14 @@ class Case:
         self._call("get_pops", self.h, C.c_int32(field), out.ctypes.data_as(C.POINTER(C.c_double)))
         return out.reshape(self.N, self.q)
 
+    def pops_noalloc(self, field: int = 0):
+        """lbf_get_pops_noalloc: the same result, staged through the library's
+        fixed device array instead of an allocation."""
+        out = np.empty(self.N * self.q)
+        self._call("get_pops_noalloc", self.h, C.c_int32(field),
+                   out.ctypes.data_as(C.POINTER(C.c_double)))
+        return out.reshape(self.N, self.q)
+
     def set_pops(self, f, field: int = 0):
         a = np.ascontiguousarray(f, dtype=np.float64).reshape(-1)
         assert a.size == self.N * self.q

@@ -66,7 +66,7 @@ def load():
         raise RuntimeError(f"oracle library missing at {LIB}; run `make -C oracle`")
     lib = C.CDLL(LIB)
     eps = [(n.replace("lbf_", "lbo_", 1), r, a) for n, r, a in abi.ENTRY_POINTS
-           if n not in ("lbf_synchronize", "lbf_device_bytes")]
+           if n not in ("lbf_synchronize", "lbf_device_bytes", "lbf_get_pops_noalloc")]
     for name, res, args in eps:
         fn = getattr(lib, name)
         fn.restype = res
