@@ -18,12 +18,69 @@ from lbflow_amd import abi
 CS2I, CS4I = 3.0, 9.0
 
 
+# Velocity sets in the reference's direction order (src/velocity_set/d2q9.rs:8-42,
+# d3q15.rs:7-57, d3q19.rs:7-67), written out here rather than read from the oracle
+_C = {
+    abi.D2Q9: [(0, 0), (1, 0), (0, 1), (-1, 0), (0, -1), (1, 1), (-1, 1), (-1, -1), (1, -1)],
+    abi.D3Q15: [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+                (1, 1, 1), (-1, -1, -1), (1, 1, -1), (-1, -1, 1), (1, -1, 1), (-1, 1, -1),
+                (-1, 1, 1), (1, -1, -1)],
+    abi.D3Q19: [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1),
+                (1, 1, 0), (-1, -1, 0), (1, 0, 1), (-1, 0, -1), (0, 1, 1), (0, -1, -1),
+                (1, -1, 0), (-1, 1, 0), (1, 0, -1), (-1, 0, 1), (0, 1, -1), (0, -1, 1)],
+}
+# weights by |c|^2
+_W = {abi.D2Q9: (4 / 9, 1 / 9, 1 / 36), abi.D3Q15: (2 / 9, 1 / 9, None, 1 / 72),
+      abi.D3Q19: (1 / 3, 1 / 18, 1 / 36)}
+
+
 def vs_tables(vs):
-    from oracle import oracle
-    t = oracle.tables(vs)
-    c = [list(map(int, row[: t["d"]])) for row in t["c"]]
-    return t["d"], t["q"], c, list(map(float, t["w"])), list(map(int, t["qbar"])), \
-        [list(map(int, r)) for r in t["qfaces"]], list(map(int, t["normal"]))
+    """(d, q, c, w, qbar, qfaces, normal) of a velocity set: qbar from c -> -c,
+    face directions with c . n_out = +1 in direction order, the inward normal
+    as the direction c = -n_out."""
+    c = [list(v) for v in _C[vs]]
+    d, q = len(c[0]), len(c)
+    w = [_W[vs][sum(x * x for x in v)] for v in c]
+    qbar = [c.index([-x for x in v]) for v in c]
+    qfaces, normal = [], []
+    for face in range(2 * d):
+        axis, sign = face // 2, (1 if face % 2 else -1)
+        qfaces.append([i for i in range(q) if c[i][axis] == sign])
+        inward = [0] * d
+        inward[axis] = -sign
+        normal.append(c.index(inward))
+    return d, q, c, w, qbar, qfaces, normal
+
+
+def mrt_rows(vs, v):
+    """Rows of the MRT moment basis at direction v: Lallemand & Luo (2000) for
+    D2Q9, d'Humieres et al. (2002) for D3Q15 and D3Q19, in the reference's row
+    order (d2q9.rs:44-156, d3q15.rs:59-363, d3q19.rs:69-548)."""
+    if vs == abi.D2Q9:
+        x, y = v
+        s = x * x + y * y
+        p = -5 + 3 * s
+        return [1, -4 + 3 * s, 4 - 10.5 * s + 4.5 * s * s, x, p * x, y, p * y, x * x - y * y, x * y]
+    x, y, z = v
+    s = x * x + y * y + z * z
+    if vs == abi.D3Q15:
+        p = -6.5 + 2.5 * s
+        return [1, -2 + s, 16 - 27.5 * s + 7.5 * s * s, x, p * x, y, p * y, z, p * z,
+                3 * x * x - s, y * y - z * z, x * y, y * z, x * z, x * y * z]
+    p, r = -9 + 5 * s, -5 + 3 * s
+    return [1, -30 + 19 * s, 12 - 26.5 * s + 10.5 * s * s, x, p * x, y, p * y, z, p * z,
+            3 * x * x - s, r * (3 * x * x - s), y * y - z * z, r * (y * y - z * z),
+            x * y, y * z, x * z, (y * y - z * z) * x, (z * z - x * x) * y, (x * x - y * y) * z]
+
+
+def mrt_matrices(vs):
+    """M[k][i] and M^-1[i][k] = M[k][i] / |row k|^2 (the rows are orthogonal)."""
+    cols = [mrt_rows(vs, v) for v in _C[vs]]
+    q = len(cols)
+    M = [[float(cols[i][k]) for i in range(q)] for k in range(q)]
+    norm = [fsum(M[k][i] * M[k][i] for i in range(q)) for k in range(q)]
+    Mi = [[M[k][i] / norm[k] for k in range(q)] for i in range(q)]
+    return M, Mi
 
 
 def fsum(xs):
@@ -44,9 +101,7 @@ class PyLattice:
         vs = spec.velocity_set
         self.d, self.q, self.c, self.w, self.qbar, self.qfaces, self.normal = vs_tables(vs)
         if spec.collision.op == abi.MRT or any(s.collision.op == abi.MRT for s in spec.scalars):
-            from oracle import oracle
-            M, Mi = oracle.mrt_matrices(vs)
-            self.M, self.Mi = M.tolist(), Mi.tolist()
+            self.M, self.Mi = mrt_matrices(vs)
         n = list(spec.shape) + [1] * (3 - spec.d)
         self.n = n
         N = n[0] * n[1] * n[2]

@@ -28,9 +28,13 @@ Parity status of each oracle function (what pins it):
   residual                reference KAT (node.rs:1152-1166) + closed form
   whole step incl. moving-wall BB, anti-BB, scalar BCs, adsorption (double
   q_ads update), Zou-He corners: bit-exact agreement with the independent
-  pure-Python restatement tests/pyref.py on tiny lattices. No test of the
-  reference itself covers these (it cannot be built here), so beyond pyref
-  they rest on the line-by-line restatement.
+  pure-Python restatement tests/pyref.py on tiny lattices (its tables and MRT
+  bases are its own, checked against the oracle's here); each of those
+  functions is also pinned by mathematics in tests/test_oracle_pins.py
+  (fixed points, conservation, a hand-stepped adsorption step).
+  Not pinned beyond the reference text: the D2Q9 / D3Q19 closed-form m_eq
+  values (d2q9.rs:247-264, d3q19.rs:550-587, including the uz + uz term) —
+  no mathematical identity fixes a quirk, so they rest on the transcription.
 """
 import json
 import os
@@ -240,9 +244,12 @@ def test_meq_quirks():
     M, _ = oracle.mrt_matrices(abi.D3Q19)
     rho, u = 1.03, np.array([0.02, -0.03, 0.04])
     meq = oracle.k_meq(abi.D3Q19, rho, u)
-    # m_9 uses uy*uy + uz + uz (d3q19.rs:575)
-    assert meq[9] == 2.0 * rho * u[0] * u[0] - rho * (u[1] * u[1] + u[2] + u[2])
     Mf = M @ oracle.k_equilibrium(abi.D3Q19, rho, u)
+    # the `uz + uz` of d3q19.rs:575,577 where the moment of f_eq has uz*uz:
+    # m_9 and m_10 = -m_9 / 2 deviate from M f_eq by exactly rho (uz^2 - 2 uz)
+    dev = rho * (u[2] * u[2] - 2.0 * u[2])
+    assert abs((meq[9] - Mf[9]) - dev) < 1e-15
+    assert abs((meq[10] - Mf[10]) + 0.5 * dev) < 1e-15
     for k in (0, 3, 5, 7, 11, 13, 14, 15):
         assert abs(meq[k] - Mf[k]) < 1e-14
     M, _ = oracle.mrt_matrices(abi.D3Q15)
@@ -477,6 +484,23 @@ def _pyref_cases():
                              adsorption=dict(adsorption_rate=0.2, q_max=1.0))]
     cases["two_scalars_mixed_bcs"] = sp
     return cases
+
+
+@pytest.mark.parametrize("vs", [abi.D2Q9, abi.D3Q15, abi.D3Q19])
+def test_python_restatement_tables_agree(vs):
+    # pyref writes its tables and MRT bases out itself (literature polynomials):
+    # they must be the oracle's numbers, bit for bit
+    import pyref
+    d, q, c, w, qbar, qfaces, normal = pyref.vs_tables(vs)
+    t = oracle.tables(vs)
+    assert d == t["d"] and q == t["q"]
+    assert np.array_equal(np.array(c), t["c"][:, :d]) and np.array_equal(w, t["w"])
+    assert np.array_equal(qbar, t["qbar"]) and np.array_equal(normal, t["normal"][: 2 * d])
+    for f in range(2 * d):
+        assert sorted(qfaces[f]) == sorted(t["qfaces"][f])
+    M, Mi = pyref.mrt_matrices(vs)
+    Mo, Mio = oracle.mrt_matrices(vs)
+    assert np.array_equal(np.array(M), Mo) and np.array_equal(np.array(Mi), Mio)
 
 
 @pytest.mark.parametrize("name", list(_pyref_cases()))
