@@ -14,6 +14,14 @@ line-by-line f64 restatement of the reference solver) on the host cores for
 a bounded sample of the same workload. The same kernel on an all-fluid, fully
 periodic 256^3 box (uniform initial velocity 0.05 along x, no bounce-back
 links) is reported as extra.all_fluid_periodic: the clean roofline point.
+
+drop_in_loop: the headline workload through lbf_run, the reference's time loop
+with residual norms and the stop rule after every step (src/momentum.rs:273-292).
+extra.other_workloads: GPU and oracle (CPU, one thread per core) rates on the
+128^2 lid-driven cavity and the 1024x512 coupled discs.
+serving: the 2,000-case serving mix as specified, on a seeded subsample run at
+full step counts (--serve-sample 2000 runs the whole mix), against sequential
+plain FP64 runs and the oracle on the host.
 """
 from __future__ import annotations
 
@@ -117,7 +125,10 @@ def sweep_bandwidth(hip, n_nodes, es, q=19, reps=10):
     return 2 * q * n_nodes * es * reps / (ms / 1e3) / 1e9
 
 
-def time_case(spec, steps, warmup, hip):
+def time_case(spec, steps, warmup, hip, loop=False):
+    """Times `steps` steps after `warmup` on the case's own stream. loop: the
+    drop-in time loop instead (lbf_run: residuals and the stop rule after
+    every step, src/momentum.rs:273-292), run for the same number of steps."""
     from lbflow_amd.engine import Case, load_library
     lib = load_library()
     lib.lbfx_case_stream.restype = C.c_void_p
@@ -130,7 +141,12 @@ def time_case(spec, steps, warmup, hip):
     e0, e1 = hip.event(), hip.event()
     t0 = time.perf_counter()
     hip.record(e0, stream)
-    case.main_steps(steps)
+    if loop:
+        # the loop stops after step index max_iterations + 1 (at most max + 2 steps)
+        done = case.run(warmup + steps - 2)
+        assert done == warmup + steps, done
+    else:
+        case.main_steps(steps)
     hip.record(e1, stream)
     case.synchronize()
     wall = time.perf_counter() - t0
@@ -140,6 +156,50 @@ def time_case(spec, steps, warmup, hip):
     mass = case.reduce(abi.REDUCE_TOTAL_MASS)
     case.close()
     return ms, wall, nonfinite, mass
+
+
+def pin_threads():
+    """One OpenMP thread per host core, bound (the reference's --pin,
+    src/cli.rs:159-190); takes effect when the oracle library loads."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+
+
+def other_cpu_baselines(hip, budget_s=8.0):
+    """The oracle and the GPU on the other single-case workloads: the 128^2
+    D2Q9 BGK lid-driven cavity (FP64, the reference's loop) and the 1024x512
+    coupled discs (FP32 populations on the GPU as the workload states, FP64 in
+    the oracle). CPU: steps grow until a run takes about budget_s."""
+    from lbflow_amd import abi, configs
+    from oracle import oracle
+    lib = oracle.load()
+    cores = lib.lbo_max_threads()
+    out = {}
+    for name, make, gpu_precs in (
+            ("cavity_128", lambda p: configs.cavity(n=128, tau=0.6, lid=0.1, precision=p),
+             (abi.F64,)),
+            ("discs_1024x512", lambda p: configs.coupled_discs(precision=p), (abi.F32, abi.F64))):
+        sp = make(abi.F64)
+        o = oracle.OracleCase(sp)
+        o.initialize_nodes()
+        o.main_steps(1)
+        steps, dt = 8, 0.0
+        while dt < budget_s / 4:
+            t0 = time.perf_counter()
+            o.main_steps(steps)
+            dt = time.perf_counter() - t0
+            if dt < budget_s / 4:
+                steps *= 4
+        o.close()
+        entry = {"cpu_mlups": round(sp.num_nodes * steps / dt / 1e6, 2), "cpu_cores": int(cores),
+                 "cpu_sample": f"oracle, {steps} steps after 1 warm-up ({dt:.1f} s)"}
+        for p in gpu_precs:
+            ms, _, _, _ = time_case(make(p), 2000, 20, hip)
+            entry[f"gpu_{abi.PRECISION_NAMES[p]}_mlups"] = round(sp.num_nodes * 2000 / ms * 1e3 / 1e6, 1)
+        print(f"[bench] {name}: " + ", ".join(f"{k} {v}" for k, v in entry.items() if "mlups" in k),
+              file=sys.stderr, flush=True)
+        out[name] = entry
+    return out
 
 
 def cpu_baseline(n=128, steps=450, threads=None):
@@ -166,19 +226,21 @@ def cpu_baseline(n=128, steps=450, threads=None):
                       f"({dt:.1f} s)"}
 
 
-def serving_cpu_baseline(params, spec_of, nodes_steps, budget_s=15.0, seed=7):
+def serving_cpu_baseline(params, spec_of, nodes_steps, budget_s=15.0, seed=7, case_s=1.5):
     """The oracle (C f64 restatement of the reference step, OpenMP on all host
-    cores) on the serving mix: cases taken in a seeded random order and run
-    one after another (create, initialise, steps, fields) until budget_s has
-    passed, then scaled to the whole mix by node updates."""
+    cores) on the serving mix: cases taken in a seeded random order until
+    budget_s has passed. Each case is created, initialised, read back, and
+    stepped for as many of its steps as fit about case_s (at least 4); its
+    full run is estimated as set-up + steps x the measured time per step. The
+    estimates are scaled to the whole mix by node updates."""
     import numpy as np
     from lbflow_amd import abi
     from oracle import oracle
     lib = oracle.load()
     cores = lib.lbo_max_threads()
     order = np.random.default_rng(seed).permutation(len(params))
-    t_sum = 0.0
-    done_updates = 0.0
+    t_sum = est_sum = 0.0
+    est_updates = ran_updates = 0.0
     ran = 0
     for i in order:
         p = params[i]
@@ -186,60 +248,83 @@ def serving_cpu_baseline(params, spec_of, nodes_steps, budget_s=15.0, seed=7):
         t0 = time.perf_counter()
         o = oracle.OracleCase(sp)
         o.initialize_nodes()
-        o.main_steps(p["steps"])
+        t1 = time.perf_counter()
+        k = 4
+        o.main_steps(k)
+        t2 = time.perf_counter()
+        per_step = (t2 - t1) / k
+        more = int(min(p["steps"] - k, max(0.0, case_s - (t2 - t0)) / max(per_step, 1e-9)))
+        if more > 0:
+            o.main_steps(more)
+            k += more
+            per_step = (time.perf_counter() - t1) / k
+        t3 = time.perf_counter()
         o.fields()
         o.close()
+        setup = (t1 - t0) + (time.perf_counter() - t3)
         t_sum += time.perf_counter() - t0
-        done_updates += sp.num_nodes * p["steps"]
+        est_sum += setup + per_step * p["steps"]
+        est_updates += sp.num_nodes * p["steps"]
+        ran_updates += sp.num_nodes * k
         ran += 1
         if t_sum >= budget_s and ran >= 3:
             break
-    total_s = t_sum * nodes_steps / done_updates
-    return {"cases_per_s": round(len(params) / total_s, 3),
-            "mlups": round(done_updates / t_sum / 1e6, 2), "cores": int(cores), "kind": "port",
-            "sample": f"oracle (C f64 restatement of the reference step, OpenMP) on {ran} cases "
-                      f"of the mix in seeded random order ({t_sum:.1f} s), scaled to the whole "
-                      f"mix by node updates"}
+    total_s = est_sum * nodes_steps / est_updates
+    return {"cases_per_s": round(len(params) / total_s, 4),
+            "mlups": round(est_updates / est_sum / 1e6, 2), "cores": int(cores), "kind": "port",
+            "sample": f"oracle (C f64 restatement of the reference step, OpenMP) on {ran} cases of "
+                      f"the mix in seeded random order, each stepped for up to {case_s} s and "
+                      f"extrapolated to its full step count ({t_sum:.1f} s measured, "
+                      f"{ran_updates:.3g} node updates run), scaled to the whole mix by node "
+                      f"updates"}
 
 
-def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100,
-                latency_sample=200, seed=5, cpu=True):
-    """Serving mix on the batched server versus sequential plain f64 runs.
-
-    Mix (configs.serving_mix): D2Q9 / D3Q19 50/50, sides log-uniform in
-    [32, 256] (D2Q9) and [32, side_3d_max] (D3Q19), tau ~ U[0.55, 1.2],
-    steps ~ U{1000, 20000} x step_scale, porosity ~ U[0.6, 0.9], FP32 or
-    FP16-shifted 50/50, periodic faces, BGK.
-      - throughput: all `count` requests submitted at once, each result
-        fetched as its case completes, served cases/s;
-        the plain baseline runs a uniform random sample of `baseline_sample`
-        cases one after another (create, FP64 steps on the default path, fields) and
-        scales its time to the whole mix;
-      - latency: `latency_sample` cases arrive as a Poisson process at 1.2x
-        the plain path's capacity; served time-to-solution is measured live,
-        the plain FIFO queue's from the measured service times of the same
-        cases (finish_i = max(arrival_i, finish_i-1) + service_i).
+def serving_leg(sample=48, plain_sample=12, two_array_sample=6, seed=5, cpu=True):
+    """The serving mix (configs.serving_mix: 2,000 cases, D2Q9 / D3Q19 50/50,
+    sides log-uniform in [32, 256], tau ~ U[0.55, 1.2], steps ~ U{1000, 20000},
+    porosity ~ U[0.6, 0.9], FP32 or FP16-shifted 50/50, periodic faces, BGK)
+    as written, on a seeded uniform subsample of `sample` of its cases run to
+    their full step counts (sample >= 2000: the whole mix). Cases/s for the
+    whole mix are the sample's times scaled by node updates (sum of nodes x
+    steps over the mix / over the sample).
+      - served: the sample submitted at once, each result fetched as its case
+        completes;
+      - plain: `plain_sample` of those cases run one after another on the
+        default FP64 path (in place, the reference's bits: create, steps,
+        fields), and `two_array_sample` on the two-array FP64 kernel;
+      - latency: the plain-sample cases arriving as a Poisson process at 1.2x
+        the default FP64 path's capacity; served time to solution is measured
+        live, the plain FIFO queue's from the measured service times
+        (finish_i = max(arrival_i, finish_i-1) + service_i);
+      - CPU: the oracle on a time-bounded sample of the mix (cpu=True).
     """
     import numpy as np
     from lbflow_amd import abi, configs
     from lbflow_amd.engine import Case
     from lbflow_amd.serve import Server
 
-    params = configs.serving_mix(count, seed=seed, side_2d=(32, 256), side_3d=(32, side_3d_max),
-                                 steps=(int(1000 * step_scale), int(20000 * step_scale)))
+    count = 2000
+    mix = configs.serving_mix(count, seed=seed)
 
-    def spec_of(p, prec=None):
+    def updates(p):
+        return float(p["side"] ** (2 if p["velocity_set"] == abi.D2Q9 else 3) * p["steps"])
+
+    def spec_of(p, prec=None, streaming=abi.STREAM_AUTO):
         return configs.serving_case(p["velocity_set"], p["side"], p["tau"], p["porosity"],
-                                    p["seed"], p["precision"] if prec is None else prec)
+                                    p["seed"], p["precision"] if prec is None else prec).replace(
+                                        streaming=streaming)
 
-    specs = [(spec_of(p), p["steps"]) for p in params]
-    nodes_steps = sum(sp.num_nodes * n for sp, n in specs)
     rng = np.random.default_rng(seed + 1)
+    idx = np.arange(count) if sample >= count else np.sort(rng.choice(count, sample, replace=False))
+    params = [mix[i] for i in idx]
+    mix_updates = sum(updates(p) for p in mix)
+    sample_updates = sum(updates(p) for p in params)
+    specs = [(spec_of(p), p["steps"]) for p in params]
 
-    def plain_service(idx):
+    def plain_service(ids, streaming):
         times = []
-        for i in idx:
-            sp = spec_of(params[i], abi.F64)  # the default FP64 path
+        for i in ids:
+            sp = spec_of(params[i], abi.F64, streaming)
             t0 = time.perf_counter()
             c = Case(sp)
             c.initialize_nodes()
@@ -249,45 +334,44 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
             times.append(time.perf_counter() - t0)
         return np.array(times)
 
-    # plain baseline on a random sample, scaled to the whole mix
-    base_idx = rng.choice(count, size=min(baseline_sample, count), replace=False)
-    print(f"[serve] plain f64 baseline on {len(base_idx)} of {count} cases", file=sys.stderr,
-          flush=True)
-    base_t = plain_service(base_idx)
-    plain_total = base_t.sum() * count / len(base_idx)
-    plain_cps = count / plain_total
-
-    # saturated throughput
-    print(f"[serve] served: {count} cases at once", file=sys.stderr, flush=True)
+    # served throughput on the sample
+    print(f"[serve] served: {len(params)} of the {count} cases at once "
+          f"({sample_updates:.3g} of {mix_updates:.3g} node updates)", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     with Server() as srv:
         tickets = [srv.submit(sp, n) for sp, n in specs]
-        # the client takes each result (rho, u) as its case completes, in
-        # submission order, while the server keeps running the others
         for t in tickets:
             srv.wait(t, 3600)
             srv.result(t)
         st = srv.stats()
     wall = time.perf_counter() - t0
-    served_cps = count / wall
+    served_mix_s = wall * mix_updates / sample_updates
 
-    # latency under Poisson arrivals at 1.2x the plain capacity
-    lat_idx = rng.choice(count, size=min(latency_sample, count), replace=False)
-    svc = plain_service(lat_idx)
-    rate = 1.2 * len(lat_idx) / svc.sum()
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(lat_idx)))
-    fin = np.zeros(len(lat_idx))
+    # plain FP64 baselines on sub-samples of the sample, scaled by node updates
+    plain_ids = rng.choice(len(params), size=min(plain_sample, len(params)), replace=False)
+    print(f"[serve] plain FP64 (default path) on {len(plain_ids)} cases", file=sys.stderr, flush=True)
+    svc = plain_service(plain_ids, abi.STREAM_AUTO)
+    plain_mix_s = svc.sum() / sum(updates(params[i]) for i in plain_ids) * mix_updates
+    two_ids = plain_ids[: min(two_array_sample, len(plain_ids))]
+    print(f"[serve] plain FP64 two-array on {len(two_ids)} cases", file=sys.stderr, flush=True)
+    svc2 = plain_service(two_ids, abi.STREAM_TWO_ARRAY)
+    two_mix_s = svc2.sum() / sum(updates(params[i]) for i in two_ids) * mix_updates
+
+    # latency: the plain-sample cases under Poisson arrivals at 1.2x plain capacity
+    rate = 1.2 * len(plain_ids) / svc.sum()
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=len(plain_ids)))
+    fin = np.zeros(len(plain_ids))
     prev = 0.0
-    for k in range(len(lat_idx)):
+    for k in range(len(plain_ids)):
         prev = max(arrivals[k], prev) + svc[k]
         fin[k] = prev
     plain_lat = fin - arrivals
-    print(f"[serve] Poisson arrivals: {len(lat_idx)} cases at {rate:.1f}/s", file=sys.stderr,
+    print(f"[serve] Poisson arrivals: {len(plain_ids)} cases at {rate:.2f}/s", file=sys.stderr,
           flush=True)
     with Server() as srv:
         start = time.perf_counter()
         tickets = []
-        for k, i in enumerate(lat_idx):
+        for k, i in enumerate(plain_ids):
             delay = arrivals[k] - (time.perf_counter() - start)
             if delay > 0:
                 time.sleep(delay)
@@ -304,26 +388,41 @@ def serving_leg(count=2000, step_scale=0.05, side_3d_max=96, baseline_sample=100
     if cpu:
         print("[serve] CPU baseline: the oracle on a time-bounded sample of the mix",
               file=sys.stderr, flush=True)
-        cpu_base = serving_cpu_baseline(params, spec_of, nodes_steps)
+        cpu_base = serving_cpu_baseline(mix, spec_of, mix_updates)
+    served_cps = count / served_mix_s
+    plain_cps = count / plain_mix_s
+    # algorithmic bytes of the served sample (2 Q s per node update) over the
+    # time the server's graph replays took (HIP events)
+    served_bytes = sum(updates(p) * 2 * (9 if p["velocity_set"] == abi.D2Q9 else 19) *
+                       (4 if p["precision"] == abi.F32 else 2) for p in params)
     return {
         "metric": "served cases/s and time to solution, serving mix",
         "cases": count,
-        "mix": f"D2Q9/D3Q19 50/50, sides log-uniform [32,256] 2-D and [32,{side_3d_max}] 3-D, "
-               f"tau U[0.55,1.2], steps U{{{int(1000 * step_scale)},{int(20000 * step_scale)}}} "
-               f"(the mix's 1,000-20,000 x {step_scale}), porosity U[0.6,0.9], FP32/FP16-shifted "
-               f"50/50, periodic, BGK, body force 1e-5",
-        "node_updates": float(nodes_steps),
-        "served_cases_per_s": round(served_cps, 2),
-        "served_mlups": round(nodes_steps / wall / 1e6, 1),
-        "plain_f64_cases_per_s": round(plain_cps, 2),
-        "plain_f64_sample": int(len(base_idx)),
+        "mix": "2,000 cases: D2Q9/D3Q19 50/50, sides log-uniform [32,256], tau U[0.55,1.2], "
+               "steps U{1000,20000}, porosity U[0.6,0.9], FP32/FP16-shifted 50/50, periodic, "
+               "BGK, body force 1e-5",
+        "sample": (f"all {count} cases" if len(params) == count else
+                   f"seeded uniform subsample of {len(params)} cases at their full step counts; "
+                   f"whole-mix times = sample times x {mix_updates / sample_updates:.2f} "
+                   f"(node updates of the mix / of the sample)"),
+        "node_updates": mix_updates,
+        "sample_node_updates": sample_updates,
+        "served_cases_per_s": round(served_cps, 3),
+        "served_mlups": round(sample_updates / wall / 1e6, 1),
+        "served_sample_wall_s": round(wall, 2),
+        "plain_f64_cases_per_s": round(plain_cps, 3),
+        "plain_f64_sample": int(len(plain_ids)),
         "throughput_ratio": round(served_cps / plain_cps, 3),
+        "plain_f64_two_array_cases_per_s": round(count / two_mix_s, 3),
+        "plain_f64_two_array_sample": int(len(two_ids)),
+        "throughput_ratio_vs_two_array": round(served_cps * two_mix_s / count, 3),
         "gpu_busy_frac": round(st["gpu_seconds"] / wall, 3),
+        "served_kernel_gbs": round(served_bytes / st["gpu_seconds"] / 1e9, 1),
         "hbm_peak_bytes": int(st["hbm_peak"]),
         "tuned_grid": int(st["tuned_grid"]),
         "tuned_tile": int(st["tuned_tile"]),
-        "poisson_rate_per_s": round(rate, 2),
-        "latency_cases": int(len(lat_idx)),
+        "poisson_rate_per_s": round(rate, 3),
+        "latency_cases": int(len(plain_ids)),
         "served_p50_s": round(float(np.percentile(lat, 50)), 4),
         "served_p99_s": round(float(np.percentile(lat, 99)), 4),
         "plain_p50_s": round(float(np.percentile(plain_lat, 50)), 4),
@@ -429,7 +528,12 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--no-all-fluid", action="store_true",
                     help="skip the all-fluid periodic box (the kernel without bounce-back)")
-    ap.add_argument("--serve-cases", type=int, default=2000)
+    ap.add_argument("--serve-sample", type=int, default=48,
+                    help="cases of the 2,000-case serving mix run in full (>= 2000: all)")
+    ap.add_argument("--serve-plain", type=int, default=12,
+                    help="cases of that sample run on the plain FP64 path")
+    ap.add_argument("--no-other", action="store_true",
+                    help="skip the cavity and coupled-discs GPU / CPU rates")
     ap.add_argument("--precision", action="store_true",
                     help="only the precision-tier report (errors vs FP64 per configuration)")
     ap.add_argument("--precision-scale", type=float, default=1.0)
@@ -446,6 +550,7 @@ def main():
                                                   [c for c in args.precision_cases.split(",") if c])}))
         return
     from lbflow_amd import abi, configs
+    pin_threads()
     hip = Hip()
     n = args.n
     N = n ** 3
@@ -500,7 +605,23 @@ def main():
                   f"({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)", file=sys.stderr, flush=True)
     head = results.get("f32") or next(iter(results.values()))
     head_name = "f32" if "f32" in results else next(iter(results))
-    serving = None if args.no_serve else serving_leg(count=args.serve_cases, cpu=not args.no_cpu)
+    # the same workload through the drop-in time loop (lbf_run): residual
+    # norms and the stop rule after every step, as the reference's loop does
+    drop_in = None
+    if head_name in ("f32", "f64_aa", "f16s", "f32s", "bf16s", "f64"):
+        name, prec, streaming, sbytes = next(v for v in variants if v[0] == head_name)
+        sp = configs.porous_channel(n=n, precision=prec, streaming=streaming)
+        ms, _, _, _ = time_case(sp, args.steps, args.warmup, hip, loop=True)
+        mlups = N * args.steps / (ms / 1e3) / 1e6
+        drop_in = dict(mlups=round(mlups, 1), ms_per_step=round(ms / args.steps, 4),
+                       frac_of_value=round(mlups / head["mlups"], 4),
+                       loop="lbf_run: residuals and stop rule every step (src/momentum.rs:273-292)")
+        print(f"[bench] {head_name} drop-in loop (lbf_run): {mlups:.0f} MLUPS", file=sys.stderr,
+              flush=True)
+    other = None if args.no_other or args.no_cpu else other_cpu_baselines(hip)
+    serving = None if args.no_serve else serving_leg(sample=args.serve_sample,
+                                                     plain_sample=args.serve_plain,
+                                                     cpu=not args.no_cpu)
     cpu = None if args.no_cpu else cpu_baseline()
     bytes_per_launch = N * head["bytes_per_site"]
     from lbflow_amd.profiling import read_traffic
@@ -532,7 +653,9 @@ def main():
                      "frac_of_sweep": round(head["achieved_gbs"] / sweep_gbs[head["storage_bytes"]], 4)},
         "cpu_baseline": cpu,
         "serving": serving,
+        "drop_in_loop": drop_in,
         "extra": dict(results, copy_gbs=round(copy_gbs, 1),
+                      other_workloads=other,
                       sweep_gbs={f"{8 * es}-bit": round(g, 1) for es, g in sweep_gbs.items()},
                       all_fluid_periodic=all_fluid or None),
     }

@@ -453,7 +453,7 @@ int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
     if (d->scalars[s].has_adsorption) b += 8ull * ld;
     if (d->scalars[s].source_kind == LBF_SOURCE_FIELD) b += 8ull * ld;
   }
-  *bytes = b + sizeof(Ctrl) + (uint64_t)((N + kBlock - 1) / kBlock) * kMaxCols * 8;
+  *bytes = b + sizeof(Ctrl) + (uint64_t)partial_rows((N + kBlock - 1) / kBlock) * kMaxCols * 8;
   return 0;
 }
 
@@ -593,7 +593,8 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   if ((rc = alloc((void**)&c->ctrl, sizeof(Ctrl)))) return cleanup(rc);
   if (launch_ctrl_set(c->ctrl, kCtrlMaxIter, -1, c->stream) != hipSuccess)  // max_iter = ~0
     return cleanup(fail(LBF_EDEVICE, "control block initialisation failed"));
-  if ((rc = alloc((void**)&c->partials, (size_t)c->grid * kMaxCols * 8))) return cleanup(rc);
+  if ((rc = alloc((void**)&c->partials, (size_t)partial_rows(c->grid) * kMaxCols * 8)))
+    return cleanup(rc);
   if (d->multiphase && (rc = alloc(&c->phi, c->ld * c->tb))) return cleanup(rc);
   if ((rc = alloc((void**)&c->red_out, 64))) return cleanup(rc);
 

@@ -12,6 +12,13 @@ constexpr uint32_t kSolidBit = 1u << 31;  // node class: solid node
 constexpr uint32_t kLinkMask = (1u << 27) - 1;  // bits 0..26: neighbour i is solid
 constexpr int kMaxCols = 12;  // residual columns: rho, u0..u2, c0..c3, nonfinite, spare
 constexpr int kHist = 64;     // residual history ring
+// step workgroups whose residual partials one folding workgroup sums
+// (misc.hip, launch_advance); the partial array holds
+// partial_rows(nblocks) rows of kMaxCols
+constexpr long long kResidualFold = 1024;
+constexpr long long partial_rows(long long nblocks) {
+  return nblocks + (nblocks + kResidualFold - 1) / kResidualFold;
+}
 
 enum Mode : int { kModePull = 0, kModeStreamOnly = 1, kModePhi = 2 };
 enum Flag : uint32_t {

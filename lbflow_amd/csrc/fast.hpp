@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(kBlock) momentum_readback_aa(KArgs a) {
        n += gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
     if (cls & kSolidBit) {
-      readback_store<Q, double>(a, n, nullptr);
+      readback_solid<Q>(a, n);
       continue;
     }
     NodePos p;

@@ -248,15 +248,22 @@ __device__ __forceinline__ uint32_t block_index(const KArgs& a) {
 // written as zeros) when a.out_aos is set; then the launch only visits
 // [out_n0, out_n1) (readback_range).
 template <int Q, class T>
-__device__ __forceinline__ void readback_store(const KArgs& a, long long n, const T* f) {
+__device__ __forceinline__ void readback_store(const KArgs& a, long long n, const T (&f)[Q]) {
   double* __restrict__ outd = (double*)a.out;
   sfor<Q>([&](auto J) {
     constexpr int j = decltype(J)::value;
     if (a.out_aos)
-      outd[(n - a.out_n0) * Q + j] = f ? (double)f[j] : 0.0;
-    else if (f)
+      outd[(n - a.out_n0) * Q + j] = (double)f[j];
+    else
       outd[j * a.ld + n] = (double)f[j];
   });
+}
+// a solid node of an AoS read-back range reads as zeros
+template <int Q>
+__device__ __forceinline__ void readback_solid(const KArgs& a, long long n) {
+  if (!a.out_aos) return;
+  double* __restrict__ outd = (double*)a.out;
+  sfor<Q>([&](auto J) { outd[(n - a.out_n0) * Q + decltype(J)::value] = 0.0; });
 }
 
 // Node range a launch visits: every node, or [out_n0, out_n1) for an AoS
@@ -300,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
        n += (long long)gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
     if (cls & kSolidBit) {
-      if (a.mode == kModeStreamOnly) readback_store<Q, T>(a, n, nullptr);
+      if (a.mode == kModeStreamOnly) readback_solid<Q>(a, n);
       continue;
     }
     T f[Q];
@@ -552,7 +559,7 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
        n += (long long)gridDim.x * kBlock) {
     const uint32_t cls = a.cls[n];
     if (cls & kSolidBit) {
-      if (a.mode == kModeStreamOnly) readback_store<Q, T>(a, n, nullptr);
+      if (a.mode == kModeStreamOnly) readback_solid<Q>(a, n);
       continue;
     }
     T gv[Q];

@@ -35,8 +35,39 @@ __global__ void __launch_bounds__(kBlock) advance_kernel(KArgs a, long long nblo
   advance_core(a, step, nblocks);
 }
 
+// First level of a step's residual reduction: workgroup g sums, column by
+// column and in a fixed order, the partial rows of the step's workgroups
+// [g * kResidualFold, (g + 1) * kResidualFold) into row nblocks + g of the
+// same array. The advancing workgroup then sums those few rows, instead of
+// reading every step workgroup's row alone (a latency-bound walk over
+// 65,536 rows at 256^3).
+__global__ void __launch_bounds__(kBlock) fold_partials(double* p, long long nblocks,
+                                                        uint32_t mask) {
+  __shared__ double lds[kBlock / 64];
+  const long long b0 = (long long)blockIdx.x * kResidualFold;
+  for (int k = 0; k < kMaxCols; k++) {
+    if (!(mask >> k & 1u)) continue;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < kResidualFold; i += kBlock) {
+      const long long b = b0 + i;
+      if (b < nblocks) s += p[b * kMaxCols + k];
+    }
+    s = block_sum(s, lds);
+    if (threadIdx.x == 0) p[(nblocks + blockIdx.x) * kMaxCols + k] = s;
+  }
+}
+
 hipError_t launch_advance(const KArgs& a, long long nblocks, hipStream_t s) {
-  hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(kBlock), 0, s, a, nblocks);
+  if ((a.flags & kFlagResiduals) && nblocks > kResidualFold) {
+    const long long folds = (nblocks + kResidualFold - 1) / kResidualFold;
+    hipLaunchKernelGGL(fold_partials, dim3((unsigned)folds), dim3(kBlock), 0, s, a.partials,
+                       nblocks, a.res_mask);
+    KArgs b = a;
+    b.partials = a.partials + nblocks * kMaxCols;
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(kBlock), 0, s, b, folds);
+  } else {
+    hipLaunchKernelGGL(advance_kernel, dim3(1), dim3(kBlock), 0, s, a, nblocks);
+  }
   return hipGetLastError();
 }
 

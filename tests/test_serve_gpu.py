@@ -222,3 +222,35 @@ def test_clients_on_several_threads_get_their_own_results():
     for ref, g in zip(refs, got):
         for a, b in zip(ref, g):
             assert np.array_equal(a, b)
+
+
+def test_served_cases_against_the_fp64_oracle():
+    # a served FP32 D2Q9 case and a served FP16-shifted D3Q19 case, in one mix,
+    # against the oracle (the reference step in f64) after the same steps:
+    # relative L2 velocity error over the fluid nodes and relative mass
+    # difference. FP16-shifted at the precision target (1e-3 / 1e-4). Plain
+    # FP32 stores f ~ w, whose deviation keeps about three digits: at this
+    # mix's velocities (about 3e-5) that is 1.3e-3 measured, bounded at 5e-3
+    from oracle import oracle
+    runs = [(case(abi.D2Q9, 48, abi.F32, seed=21, tau=0.7), 120, 5e-3, 1e-5),
+            (case(abi.D3Q19, 24, abi.F16_SHIFTED, seed=22, tau=0.9), 80, 1e-3, 1e-4)]
+    with Server(max_steps_per_round=16) as srv:
+        tickets = [srv.submit(sp, steps) for sp, steps, _, _ in runs]
+        for t in tickets:
+            srv.wait(t, 120)
+        served = [srv.result(t)[:2] for t in tickets]
+    errs = []
+    for (sp, steps, tol_u, tol_m), (rho, u) in zip(runs, served):
+        o = oracle.OracleCase(sp.replace(precision=abi.F64))
+        o.initialize_nodes()
+        o.main_steps(steps)
+        ro, uo = o.fields()
+        fluid = sp.node_type == 0
+        e_u = np.linalg.norm((u - uo)[fluid]) / np.linalg.norm(uo[fluid])
+        e_m = abs(rho[fluid].sum() - ro[fluid].sum()) / ro[fluid].sum()
+        print(f"[serve-oracle] vs={sp.velocity_set} prec={sp.precision}: L2 u {e_u:.2e}, "
+              f"mass {e_m:.2e}", flush=True)
+        errs.append((np.isfinite(u).all(), e_u, tol_u, e_m, tol_m))
+        o.close()
+    for finite, e_u, tol_u, e_m, tol_m in errs:
+        assert finite and e_u < tol_u and e_m < tol_m
