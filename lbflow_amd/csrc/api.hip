@@ -183,6 +183,12 @@ struct ScalarDev {
   lbf_scalar_desc desc{};
 };
 
+// Steps per captured batch, and the largest case that replays batches (a
+// 2^21-node step takes about 60 us in FP32, so the few microseconds of launch
+// latency per kernel matter only below that).
+constexpr uint32_t kGraphSteps = 16;
+constexpr long long kGraphMaxNodes = 1ll << 21;
+
 struct lbf_case {
   lbf_case_desc desc{};
   VsInfo v{};
@@ -216,6 +222,15 @@ struct lbf_case {
   double w[27] = {};
   bool frozen = false;
   bool needs_moments = false;  // some condition reads stored moments every step
+  // captured batches of kGraphSteps steps (cases of at most kGraphMaxNodes
+  // nodes, where launch latency bounds the step), keyed by step flags
+  bool graphs_on = false;
+  struct StepGraph {
+    uint32_t flags;
+    bool every_last;
+    hipGraphExec_t exec;
+  } graphs[4];
+  int ngraphs = 0;
   uint32_t face_active = 0;
   // host copies of the node classes
   std::vector<uint8_t> h_type;  // node types (host copy)
@@ -253,8 +268,14 @@ hipError_t dev_alloc(lbf_case* c, void** p, size_t bytes) {
   return hipMalloc(p, bytes);
 }
 
+void drop_step_graphs(lbf_case* c) {
+  for (int i = 0; i < c->ngraphs; i++) (void)hipGraphExecDestroy(c->graphs[i].exec);
+  c->ngraphs = 0;
+}
+
 void free_case(lbf_case* c) {
   if (!c) return;
+  drop_step_graphs(c);
   const bool pooled = c->stream && use_pool(c->device);
   auto F = [&](void* p) {
     if (!p) return;
@@ -547,6 +568,10 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   // two-array kernels keep the round-robin order: there runs of 1-32 blocks
   // cut HBM reads by up to a fifth yet cost 0-6 % in time (measured).
   c->xcd_group = (c->aa && c->grid % 32 == 0) ? 4 : 0;
+  {
+    const char* env = getenv("LBF_STEP_GRAPHS");
+    c->graphs_on = N <= kGraphMaxNodes && !(env && env[0] == '0');
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
   // buffers are cleared on the case stream; the stream is synchronised
@@ -812,6 +837,59 @@ int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
   return 0;
 }
 
+// The captured batch of kGraphSteps steps for these step flags (captured on
+// first use from an even, non-fresh step: the in-place launches alternate by
+// parity). Replays enqueue exactly the launches enqueue_step would.
+int step_graph(lbf_case* c, uint32_t flags, bool every_last, hipGraphExec_t* out) {
+  for (int i = 0; i < c->ngraphs; i++)
+    if (c->graphs[i].flags == flags && c->graphs[i].every_last == every_last) {
+      *out = c->graphs[i].exec;
+      return 0;
+    }
+  if (c->ngraphs == (int)(sizeof(c->graphs) / sizeof(c->graphs[0]))) drop_step_graphs(c);
+  const uint64_t step0 = c->host_step;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = 0;
+  for (uint32_t k = 0; k < kGraphSteps && !rc; k++) rc = enqueue_step(c, flags, every_last);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(c->stream, &g);
+  c->host_step = step0;  // capture enqueued nothing
+  if (rc || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc ? rc : fail(LBF_EDEVICE, "step capture failed: %s", hipGetErrorString(e));
+  }
+  hipGraphExec_t x = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (ei != hipSuccess) return fail(LBF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+  c->graphs[c->ngraphs++] = {flags, every_last, x};
+  *out = x;
+  return 0;
+}
+
+// nsteps steps: captured batches where the case replays them (from an even,
+// non-fresh step, keeping the last step of a batch that stores moments out of
+// them), single launches otherwise.
+int enqueue_steps(lbf_case* c, uint64_t nsteps, uint32_t flags, bool every_last) {
+  for (uint64_t k = 0; k < nsteps;) {
+    const uint64_t left = nsteps - k;
+    if (c->graphs_on && !c->aa_fresh && (c->host_step & 1) == 0 &&
+        (every_last ? left >= kGraphSteps : left > kGraphSteps)) {
+      hipGraphExec_t g = nullptr;
+      int rc = step_graph(c, flags, every_last, &g);
+      if (rc) return rc;
+      HIPCHK(hipGraphLaunch(g, c->stream));
+      c->host_step += kGraphSteps;
+      k += kGraphSteps;
+    } else {
+      int rc = enqueue_step(c, flags, every_last || k + 1 == nsteps);
+      if (rc) return rc;
+      k++;
+    }
+  }
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -848,11 +926,7 @@ int lbf_main_steps(lbf_case* c, uint32_t nsteps, uint32_t flags) {
     HIPCHK(launch_ctrl_set(c->ctrl, kCtrlMaxIter, -1, c->stream));
     c->may_have_stopped = true;
   }
-  for (uint32_t k = 0; k < nsteps; k++) {
-    int rc = enqueue_step(c, flags, k + 1 == nsteps);
-    if (rc) return rc;
-  }
-  return 0;
+  return enqueue_steps(c, nsteps, flags, false);
 }
 
 int lbf_synchronize(lbf_case* c) {
@@ -874,8 +948,7 @@ int lbf_run(lbf_case* c, uint64_t max_iterations, uint64_t* steps_done) {
     const uint64_t limit = max_iterations + 2;
     const uint64_t left = h.step >= limit ? 1 : limit - h.step;
     const uint64_t n = std::min<uint64_t>(chunk, left);
-    for (uint64_t k = 0; k < n; k++)
-      if ((rc = enqueue_step(c, flags, true))) return rc;
+    if ((rc = enqueue_steps(c, n, flags, true))) return rc;
     if ((rc = read_ctrl(c, &h))) return rc;
     if (h.stopped) break;
     chunk = std::min<uint64_t>(chunk * 2, 1024);
@@ -1164,6 +1237,7 @@ int lbf_get_link_tables(lbf_case* c, uint8_t* node_type, uint32_t* bb_link_mask,
 extern "C" int lbfx_set_block_group(lbf_case* c, int g) {
   if (!c || g < 0 || (g > 0 && c->grid % (8 * g) != 0)) return -1;
   c->xcd_group = g;
+  drop_step_graphs(c);  // captured launches carry the block order
   return 0;
 }
 
