@@ -327,7 +327,9 @@ struct lbf_server {
         return err(LBF_ENOMEM, "pinned tile staging (%zu entries)", total);
       }
     }
-    size_t off = 0;
+    // grow the device tile lists first, before this upload queues any copy:
+    // no list is freed while the stream still has work (the previous round's
+    // replay was waited for)
     for (int t = 0; t < kTypes; t++) {
       if (count[t] > tiles_cap[t]) {
         drop_graphs();  // graphs hold the old tile pointers
@@ -339,6 +341,9 @@ struct lbf_server {
           return err(LBF_ENOMEM, "tile list (%zu entries)", count[t]);
         }
       }
+    }
+    size_t off = 0;
+    for (int t = 0; t < kTypes; t++) {
       size_t k = off;
       for (int i = 0; i < kServeMaxSlots; i++) {
         Request* r = slot_req[i];
