@@ -44,12 +44,15 @@ typedef struct lbf_server_config {
   uint64_t hbm_budget_bytes;    /* device bytes resident cases may hold (0 = 80 % of
                                    the device's free memory at creation)             */
   uint32_t max_steps_per_round; /* steps per graph replay, power of two <= 256 (0 = 64) */
-  int32_t autotune;             /* 1: choose the batched launch's workgroup count and
+  int32_t autotune;             /* 1: choose the batched launches' workgroup cap and
                                    tile size by measurement per mix of case types;
                                    finished choices are kept for the process and
                                    reused by later servers                          */
-  uint32_t grid;                /* workgroups per batched launch when autotune = 0
-                                   (0 = 2048)                                       */
+  uint32_t grid;                /* workgroup cap of a batched launch when autotune =
+                                   0: a launch takes min(grid, its tile count rounded
+                                   up to a power of two) workgroups, each walking
+                                   its share of the tiles (0 = no cap: one tile per
+                                   workgroup)                                        */
   uint32_t tile;                /* nodes per tile (= threads per workgroup) when
                                    autotune = 0: 128, 256 or 512 (0 = 256)          */
 } lbf_server_config;
@@ -76,7 +79,8 @@ typedef struct lbf_request_info {
 typedef struct lbf_server_stats {
   uint64_t submitted, completed, failed, queued, resident;
   uint64_t rounds;            /* graph replays                                   */
-  uint64_t graphs_built;      /* distinct (type mix, K, grid, tile) graphs captured */
+  uint64_t graphs_built;      /* graphs captured (one per launch shape of a mix:
+                                 types, K, cap, tile, per-class tile buckets)   */
   uint64_t hbm_in_use, hbm_peak, hbm_budget;
   double node_updates;        /* sum of nodes * steps over completed requests     */
   double first_submit, last_done; /* seconds since server creation               */
@@ -86,7 +90,7 @@ typedef struct lbf_server_stats {
   double mlups;               /* node_updates / (last_done - first_submit) / 1e6  */
   double prep_seconds;        /* case creation + initialisation, summed over helper threads */
   double retire_seconds;      /* result download + release, summed over helper threads */
-  uint32_t tuned_grid;        /* workgroup count the autotuner settled on last    */
+  uint32_t tuned_grid;        /* workgroup cap the autotuner settled on last      */
   uint32_t tuned_tile;        /* tile size the autotuner settled on last          */
 } lbf_server_stats;
 

@@ -9,9 +9,13 @@ namespace lbf {
 using LaunchFn = hipError_t (*)(const KArgs&, int grid, hipStream_t);
 
 struct ServeSlot;  // serve.hpp
-// batched step over a tile list (serving tier): slots, tiles, &ntiles, grid
-using ServeLaunchFn = hipError_t (*)(const ServeSlot*, const uint32_t*, const uint32_t*, int,
-                                     hipStream_t);
+// batched step of one parity class of a tile list (serving tier): slots,
+// tiles, counts (class 0, class 1), class, odd, grid
+using ServeLaunchFn = hipError_t (*)(const ServeSlot*, const uint32_t*, const uint32_t*, int, int,
+                                     int, hipStream_t);
+// last steps over the same list: slots, tiles, counts, last-step flags, type bit, grid
+using ServeLastFn = hipError_t (*)(const ServeSlot*, const uint32_t*, const uint32_t*,
+                                   const uint32_t*, uint32_t, int, hipStream_t);
 
 struct Launchers {
   LaunchFn mom_step = nullptr;  // fused pull-stream / faces / collide
@@ -23,6 +27,7 @@ struct Launchers {
   LaunchFn aa_readback = nullptr;
   // [BGK, TRT][tile size 128, 256, 512] (D2Q9 / D3Q19, AA storages)
   ServeLaunchFn serve[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
+  ServeLastFn serve_last[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
 };
 
 // precision: LBF_F64 .. LBF_BF16_SHIFTED; returns false when not built

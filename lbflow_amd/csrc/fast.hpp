@@ -482,12 +482,13 @@ __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T sc, T is
 }
 
 // One node of the AA-pattern fused step (OP 0 BGK, 1 TRT, 2 MRT; ODD selects
-// the step parity). `step` is the case's completed-step count, `fresh` marks
-// an array that holds post-stream f in natural slots. MOM: store moments (and
-// accumulate residual terms into acc) as the flags ask.
+// the step parity) with its class word `cls` (kSolidBit past the last node).
+// `step` is the case's completed-step count, `fresh` marks an array that holds
+// post-stream f in natural slots. MOM: store moments (and accumulate residual
+// terms into acc) as the flags ask.
 template <class L, class St, int OP, bool ODD, bool MOM>
-__device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned long long step,
-                                        bool fresh, double* acc) {
+__device__ __forceinline__ void aa_node_cls(const KArgs& a, uint32_t n, uint32_t cls,
+                                            unsigned long long step, bool fresh, double* acc) {
 #pragma clang fp contract(fast)
   using T = typename St::T;
   using S = typename St::S;
@@ -507,9 +508,7 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
   const T omega = T(a.omega), omega_p = T(a.omega_p), wp = T(a.wp), wm = T(a.wm),
           coeff_b = T(a.coeff_b);
   const T sc = T(a.scale), isc = T(a.inv_scale);
-  const uint32_t N = (uint32_t)a.N;
   const uint32_t ld = (uint32_t)a.ld;
-  const uint32_t cls = n < N ? a.cls[n] : kSolidBit;
   if (!(cls & kSolidBit)) {
     NodePos p;
     node_pos<L>(a, n, true, p);
@@ -712,6 +711,17 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
       }
     });
   }
+}
+
+// class word of node n of a case (kSolidBit past its last node)
+__device__ __forceinline__ uint32_t node_class(const KArgs& a, uint32_t n) {
+  return n < (uint32_t)a.N ? a.cls[n] : kSolidBit;
+}
+
+template <class L, class St, int OP, bool ODD, bool MOM>
+__device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned long long step,
+                                        bool fresh, double* acc) {
+  aa_node_cls<L, St, OP, ODD, MOM>(a, n, node_class(a, n), step, fresh, acc);
 }
 
 // AA-pattern fused step over one case. MOM: this launch may store moments /

@@ -75,6 +75,12 @@ static void serve_launchers(Launchers* out) {
     out->serve[1][0] = serve_launch<L, St, 1, kTileSizes[0]>;
     out->serve[1][1] = serve_launch<L, St, 1, kTileSizes[1]>;
     out->serve[1][2] = serve_launch<L, St, 1, kTileSizes[2]>;
+    out->serve_last[0][0] = serve_launch_last<L, St, 0, kTileSizes[0]>;
+    out->serve_last[0][1] = serve_launch_last<L, St, 0, kTileSizes[1]>;
+    out->serve_last[0][2] = serve_launch_last<L, St, 0, kTileSizes[2]>;
+    out->serve_last[1][0] = serve_launch_last<L, St, 1, kTileSizes[0]>;
+    out->serve_last[1][1] = serve_launch_last<L, St, 1, kTileSizes[1]>;
+    out->serve_last[1][2] = serve_launch_last<L, St, 1, kTileSizes[2]>;
   }
 }
 

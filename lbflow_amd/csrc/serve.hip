@@ -43,21 +43,42 @@
 namespace lbf {
 
 // Ends one step of every active slot: the case's step counter and the
-// remaining count advance together (one thread per slot).
-__global__ void __launch_bounds__(kServeMaxSlots) serve_advance(ServeSlot* slots) {
+// remaining count advance together (one thread per slot). Then *last_flags =
+// the type bits of the slots that have one step left, which the next step's
+// serve_last launches read.
+__global__ void __launch_bounds__(kServeMaxSlots) serve_advance(ServeSlot* slots,
+                                                                uint32_t* last_flags) {
+  __shared__ uint32_t bits;
+  if (threadIdx.x == 0) bits = 0;
+  __syncthreads();
   ServeSlot& s = slots[threadIdx.x];
   if (s.remaining > 0) {
     s.a.ctrl->step += 1;
+    s.step += 1;
     s.remaining -= 1;
+    if (s.remaining == 1) atomicOr(&bits, s.type_bit);
   }
+  __syncthreads();
+  if (threadIdx.x == 0) *last_flags = bits;
 }
 
 namespace {
 
 // case types: lattice (D2Q9, D3Q19) x storage (F32, F32S, F16S, BF16S) x op (BGK, TRT)
 constexpr int kTypes = 16;
+// device counters: per type the tile counts of parity classes 0 and 1, then
+// the last-step flags
+constexpr int kCounters = 2 * kTypes + 1;
+constexpr int kLastFlags = 2 * kTypes;
 constexpr int kHelpers = 8;  // threads that set up and release cases
-constexpr uint32_t kGridCandidates[] = {512, 1024, 2048, 4096, 8192};
+// caps on the workgroups of one batched launch the autotuner tries; a launch
+// takes min(cap, its tile count rounded up to a power of two), so the largest
+// cap runs one tile per workgroup
+constexpr uint32_t kGridCandidates[] = {2048, 8192, 32768, 1u << 22};
+// workgroups of a last-step launch at most (it usually returns at once)
+constexpr uint32_t kLastGridMax = 2048;
+// distinct graphs kept; past this the cache is emptied
+constexpr size_t kMaxGraphs = 256;
 constexpr int kNumCandidates = sizeof(kGridCandidates) / sizeof(kGridCandidates[0]);
 
 int tile_index_of(uint32_t tile) {
@@ -97,6 +118,7 @@ struct Request {
   lbf_case* c = nullptr;
   int slot = -1;
   long long remaining = 0;
+  unsigned long long step0 = 0;  // the case's step count when it was activated
   std::vector<double> rho, u, pops;
 };
 
@@ -163,10 +185,17 @@ struct lbf_server {
   uint32_t* h_tiles = nullptr;
   size_t h_tiles_cap = 0;
   ServeLaunchFn launch[kTypes][kNumTileSizes] = {};
+  ServeLastFn launch_last[kTypes][kNumTileSizes] = {};
   int cfg_tile = kDefaultTileIdx;  // tile size index when autotune is off
   int cur_tile = kDefaultTileIdx;  // tile size index of the tile lists on the device
   bool dirty = true;
-  std::map<uint64_t, hipGraphExec_t> graphs;
+  // per type and parity class: tile count of the tables on the device and its
+  // power-of-two bucket (0: no tile), which sizes that class's launches; the
+  // bucket grows past the count at once and shrinks when the count falls to a
+  // quarter of it, so graphs stay valid while the mix drifts
+  uint32_t count_of[kTypes][2] = {};
+  uint32_t bucket[kTypes][2] = {};
+  std::map<std::vector<uint32_t>, hipGraphExec_t> graphs;
   std::map<uint32_t, Tune> tune;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -285,11 +314,14 @@ struct lbf_server {
     if (rc) return rc;
     r->slot = slot;
     r->remaining = (long long)r->steps;
+    r->step0 = info.step;
     slot_req[slot] = r;
     ServeSlot& s = h_slots[slot];
     s.a = info.args;
     s.remaining = r->remaining;
     s.fresh_step = info.fresh ? info.step : ~0ull;
+    s.step = info.step;
+    s.type_bit = 1u << r->type;
     dirty = true;
     return 0;
   }
@@ -304,6 +336,11 @@ struct lbf_server {
   void drop_graphs() {
     for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
     graphs.clear();
+  }
+
+  // parity of the case's step count now (its class in the tile lists)
+  static int parity_of(const Request* r) {
+    return (int)((r->step0 + r->steps - (unsigned long long)r->remaining) & 1u);
   }
 
   // tiles of a case at the current tile size (the last one may be partial)
@@ -342,16 +379,33 @@ struct lbf_server {
         }
       }
     }
+    // per type: the tiles of the cases at an even step count, then those at an
+    // odd one (the graph launches each class with its parity)
     size_t off = 0;
+    uint32_t last = 0;
     for (int t = 0; t < kTypes; t++) {
       size_t k = off;
-      for (int i = 0; i < kServeMaxSlots; i++) {
-        Request* r = slot_req[i];
-        if (!r || r->type != t) continue;
-        const uint32_t nb = tiles_of(r);
-        for (uint32_t b = 0; b < nb; b++) h_tiles[k++] = ((uint32_t)i << (32 - kServeSlotBits)) | b;
+      for (int cls = 0; cls < 2; cls++) {
+        const size_t k0 = k;
+        for (int i = 0; i < kServeMaxSlots; i++) {
+          Request* r = slot_req[i];
+          if (!r || r->type != t || parity_of(r) != cls) continue;
+          const uint32_t nb = tiles_of(r);
+          for (uint32_t b = 0; b < nb; b++)
+            h_tiles[k++] = ((uint32_t)i << (32 - kServeSlotBits)) | b;
+          if (h_slots[i].remaining == 1) last |= 1u << t;
+        }
+        h_ntiles[2 * t + cls] = (uint32_t)(k - k0);
+        count_of[t][cls] = (uint32_t)(k - k0);
+        uint32_t& b = bucket[t][cls];
+        const uint32_t c = count_of[t][cls];
+        if (!c)
+          b = 0;
+        else if (c > b || (uint64_t)c * 4 <= b) {
+          b = 1;
+          while (b < c) b <<= 1;
+        }
       }
-      h_ntiles[t] = (uint32_t)count[t];
       if (count[t]) {
         const hipError_t e =
             hipMemcpyAsync(d_tiles[t], h_tiles + off, count[t] * 4, hipMemcpyHostToDevice, stream);
@@ -359,7 +413,8 @@ struct lbf_server {
       }
       off = k;
     }
-    hipError_t e = hipMemcpyAsync(d_ntiles, h_ntiles, kTypes * 4, hipMemcpyHostToDevice, stream);
+    h_ntiles[kLastFlags] = last;
+    hipError_t e = hipMemcpyAsync(d_ntiles, h_ntiles, kCounters * 4, hipMemcpyHostToDevice, stream);
     if (e == hipSuccess)
       e = hipMemcpyAsync(d_slots, h_slots, sizeof(ServeSlot) * kServeMaxSlots,
                          hipMemcpyHostToDevice, stream);
@@ -368,9 +423,14 @@ struct lbf_server {
     return 0;
   }
 
-  int get_graph(uint32_t mask, int K, uint32_t grid, int tile, hipGraphExec_t* out) {
-    const uint64_t key =
-        (uint64_t)mask | ((uint64_t)K << 16) | ((uint64_t)grid << 32) | ((uint64_t)tile << 56);
+  // Graph of K steps of the resident mix: per type the launches of its
+  // non-empty parity classes (workgroups min(cap, bucket)), the last-step
+  // launch for the types with a case ending within these K steps (`last`),
+  // then serve_advance; keyed by everything that shapes it.
+  int get_graph(uint32_t mask, uint32_t last, int K, uint32_t cap, int tile, hipGraphExec_t* out) {
+    std::vector<uint32_t> key = {mask, last, (uint32_t)K, cap, (uint32_t)tile};
+    for (int t = 0; t < kTypes; t++)
+      if (mask >> t & 1u) key.insert(key.end(), {bucket[t][0], bucket[t][1]});
     auto it = graphs.find(key);
     if (it != graphs.end()) {
       *out = it->second;
@@ -396,13 +456,27 @@ struct lbf_server {
     std::unique_lock<std::shared_mutex> hl(hip_mu);
     hipError_t eb = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
     if (eb != hipSuccess) return err(LBF_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(eb));
+    if (graphs.size() >= kMaxGraphs) drop_graphs();
+    // step k of the graph: class c of every type is at parity (c + k) & 1
     bool ok = true;
     for (int k = 0; k < K && ok; k++) {
-      for (int t = 0; t < kTypes && ok; t++)
-        if (mask >> t & 1u)
-          ok = launch[t][tile](d_slots, d_tiles[t], d_ntiles + t, (int)grid, stream) == hipSuccess;
+      for (int t = 0; t < kTypes && ok; t++) {
+        if (!(mask >> t & 1u)) continue;
+        for (int cls = 0; cls < 2 && ok; cls++)
+          if (bucket[t][cls])
+            ok = launch[t][tile](d_slots, d_tiles[t], d_ntiles + 2 * t, cls, (cls + k) & 1,
+                                 (int)std::min(cap, bucket[t][cls]), stream) == hipSuccess;
+        if (ok && (last >> t & 1u)) {
+          const uint32_t all = std::max(bucket[t][0], bucket[t][1]);
+          ok = launch_last[t][tile](d_slots, d_tiles[t], d_ntiles + 2 * t,
+                                    d_ntiles + kLastFlags, 1u << t,
+                                    (int)std::min({cap, kLastGridMax, 2 * all}),
+                                    stream) == hipSuccess;
+        }
+      }
       if (ok) {
-        hipLaunchKernelGGL(serve_advance, dim3(1), dim3(kServeMaxSlots), 0, stream, d_slots);
+        hipLaunchKernelGGL(serve_advance, dim3(1), dim3(kServeMaxSlots), 0, stream, d_slots,
+                           d_ntiles + kLastFlags);
         ok = hipGetLastError() == hipSuccess;
       }
     }
@@ -616,11 +690,14 @@ struct lbf_server {
       int K = 1;
       while (2 * K <= (long long)cfg.max_steps_per_round && K < max_rem) K *= 2;
       double work = 0;  // node updates this replay performs
+      uint32_t last = 0;  // types with a case whose last step falls in this replay
       for (int i = 0; i < kServeMaxSlots; i++)
-        if (slot_req[i])
+        if (slot_req[i]) {
           work += (double)slot_req[i]->nodes * (double)std::min<long long>(K, slot_req[i]->remaining);
+          if (slot_req[i]->remaining <= K) last |= 1u << slot_req[i]->type;
+        }
       hipGraphExec_t g = nullptr;
-      if ((rc = get_graph(mask, K, grid, tile, &g))) {
+      if ((rc = get_graph(mask, last, K, grid, tile, &g))) {
         fail_all_resident(rc);
         continue;
       }
@@ -643,10 +720,13 @@ struct lbf_server {
       for (int i = 0; i < kServeMaxSlots; i++) {
         Request* r = slot_req[i];
         if (!r) continue;
-        r->remaining = std::max<long long>(0, r->remaining - K);  // as serve_advance left it
+        const long long ran = std::min<long long>(K, r->remaining);
+        r->remaining -= ran;  // as serve_advance left it
         h_slots[i].remaining = r->remaining;
+        h_slots[i].step += (unsigned long long)ran;
         if (r->remaining == 0) retire(r, 0);
       }
+      if (K & 1) dirty = true;  // every case changed parity: rebuild the classes
     }
   }
 
@@ -741,7 +821,7 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
   if (!s->cfg.max_slots || s->cfg.max_slots > (uint32_t)kServeMaxSlots) s->cfg.max_slots = kServeMaxSlots;
   if (!s->cfg.max_steps_per_round) s->cfg.max_steps_per_round = 64;
   s->cfg.max_steps_per_round = std::min<uint32_t>(s->cfg.max_steps_per_round, 256);
-  if (!s->cfg.grid) s->cfg.grid = 2048;
+  if (!s->cfg.grid) s->cfg.grid = kGridCandidates[kNumCandidates - 1];
   if (!s->cfg.tile) s->cfg.tile = (uint32_t)kTileSizes[kDefaultTileIdx];
   s->cfg_tile = tile_index_of(s->cfg.tile);
   if (s->cfg_tile < 0) return err(LBF_EINVAL, "tile %u: use 128, 256 or 512", s->cfg.tile);
@@ -755,13 +835,13 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
   s->t0 = std::chrono::steady_clock::now();
   bool ok = create_worker_stream(s->device, &s->stream) == hipSuccess &&
             hipMalloc((void**)&s->d_slots, sizeof(ServeSlot) * kServeMaxSlots) == hipSuccess &&
-            hipMalloc((void**)&s->d_ntiles, kTypes * 4) == hipSuccess &&
+            hipMalloc((void**)&s->d_ntiles, kCounters * 4) == hipSuccess &&
             hipHostMalloc((void**)&s->h_slots, sizeof(ServeSlot) * kServeMaxSlots) == hipSuccess &&
-            hipHostMalloc((void**)&s->h_ntiles, kTypes * 4) == hipSuccess &&
+            hipHostMalloc((void**)&s->h_ntiles, kCounters * 4) == hipSuccess &&
             hipEventCreate(&s->ev0) == hipSuccess && hipEventCreate(&s->ev1) == hipSuccess;
   if (ok) {
     memset(s->h_slots, 0, sizeof(ServeSlot) * kServeMaxSlots);
-    memset(s->h_ntiles, 0, kTypes * 4);
+    memset(s->h_ntiles, 0, kCounters * 4);
     for (int t = 0; t < kTypes && ok; t++) {
       s->tiles_cap[t] = 1024;
       ok = hipMalloc((void**)&s->d_tiles[t], s->tiles_cap[t] * 4) == hipSuccess;
@@ -769,14 +849,17 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
       Launchers L{};
       const bool have = lat ? launchers_d3q19(prec, &L) : launchers_d2q9(prec, &L);
       for (int k = 0; k < kNumTileSizes; k++) {
-        ok = ok && have && L.serve[op][k];
-        if (ok) s->launch[t][k] = L.serve[op][k];
+        ok = ok && have && L.serve[op][k] && L.serve_last[op][k];
+        if (ok) {
+          s->launch[t][k] = L.serve[op][k];
+          s->launch_last[t][k] = L.serve_last[op][k];
+        }
       }
     }
     ok = ok &&
          hipMemcpyAsync(s->d_slots, s->h_slots, sizeof(ServeSlot) * kServeMaxSlots,
                         hipMemcpyHostToDevice, s->stream) == hipSuccess &&
-         hipMemcpyAsync(s->d_ntiles, s->h_ntiles, kTypes * 4, hipMemcpyHostToDevice, s->stream) ==
+         hipMemcpyAsync(s->d_ntiles, s->h_ntiles, kCounters * 4, hipMemcpyHostToDevice, s->stream) ==
              hipSuccess &&
          hipStreamSynchronize(s->stream) == hipSuccess;
   }

@@ -77,14 +77,15 @@ def test_result_is_bitwise_the_same_alone_and_in_a_mix():
         assert np.array_equal(a, b)
 
 
-@pytest.mark.parametrize("tile", [128, 256, 512])
-def test_each_tile_size_gives_the_single_case_result(tile):
-    # 37^2 and 19^3 nodes leave a partial last tile at every size
+@pytest.mark.parametrize("tile,grid", [(128, 0), (256, 0), (512, 0), (256, 2)])
+def test_each_tile_size_gives_the_single_case_result(tile, grid):
+    # 37^2 and 19^3 nodes leave a partial last tile at every size; grid 0 runs
+    # one tile per workgroup, grid 2 makes each workgroup walk many tiles
     specs = [(case(abi.D2Q9, 37, abi.F32, seed=21), 13),
              (case(abi.D3Q19, 19, abi.F16_SHIFTED, seed=22), 11)]
-    print(f"[serve-tile] tile={tile}", flush=True)
+    print(f"[serve-tile] tile={tile} grid={grid}", flush=True)
     refs = [single(sp, n) for sp, n in specs]
-    with Server(autotune=False, tile=tile, max_steps_per_round=4) as srv:
+    with Server(autotune=False, tile=tile, grid=grid, max_steps_per_round=4) as srv:
         ts = [srv.submit(sp, n, keep_pops=True) for sp, n in specs]
         for t, ref in zip(ts, refs):
             srv.wait(t, 120)
@@ -113,10 +114,10 @@ def test_autotuner_searches_tile_sizes_and_caches_its_choice():
         srv.drain(120)
         st = srv.stats()
         got = [srv.result(t) for t in ts]
-    # five workgroup counts at 256-node tiles, then the 128 and 512 tiles
-    assert st["tuned_grid"] in (512, 1024, 2048, 4096, 8192)
+    # four workgroup caps at 256-node tiles, then the 128 and 512 tiles
+    assert st["tuned_grid"] in (2048, 8192, 32768, 1 << 22)
     assert st["tuned_tile"] in (128, 256, 512)
-    assert st["graphs_built"] >= 7
+    assert st["graphs_built"] >= 6
     # the launch shape changed between rounds; the results did not
     for ref, g in zip(refs, got):
         for a, b in zip(ref, g):
@@ -126,7 +127,9 @@ def test_autotuner_searches_tile_sizes_and_caches_its_choice():
         t = srv.submit(specs[0][0], specs[0][1])
         srv.wait(t, 60)
         st2 = srv.stats()
-    assert st2["graphs_built"] == 1 and st2["tuned_tile"] == 0  # no search
+    # no search; with one step per round the case's parity class alternates
+    # (two graphs) and its last round adds the last-step launch (a third)
+    assert st2["graphs_built"] <= 3 and st2["tuned_tile"] == 0
 
 
 def test_admission_respects_the_hbm_budget():
