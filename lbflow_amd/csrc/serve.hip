@@ -25,6 +25,7 @@
 #include <cstdarg>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -196,6 +197,9 @@ struct lbf_server {
   uint32_t count_of[kTypes][2] = {};
   uint32_t bucket[kTypes][2] = {};
   std::map<std::vector<uint32_t>, hipGraphExec_t> graphs;
+  // LBF_SERVE_EAGER=1: rounds enqueue their launches directly instead of
+  // replaying captured graphs (same kernels, same order; for tracing tools)
+  bool eager = false;
   std::map<uint32_t, Tune> tune;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
@@ -457,7 +461,31 @@ struct lbf_server {
     hipError_t eb = hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed);
     if (eb != hipSuccess) return err(LBF_EDEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(eb));
     if (graphs.size() >= kMaxGraphs) drop_graphs();
-    // step k of the graph: class c of every type is at parity (c + k) & 1
+    const bool ok = enqueue_round(mask, last, K, cap, tile);
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &g);
+    if (!ok || e != hipSuccess || !g) {
+      if (g) (void)hipGraphDestroy(g);
+      return err(LBF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return err(LBF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+    graphs[key] = x;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      graphs_built++;
+    }
+    *out = x;
+    return 0;
+  }
+
+  // The launches of one round on the worker stream (captured by get_graph, or
+  // enqueued as they are with LBF_SERVE_EAGER): step k of the round runs, per
+  // type, its non-empty parity classes (class c at parity (c + k) & 1), the
+  // last-step launch for the types in `last`, then serve_advance.
+  bool enqueue_round(uint32_t mask, uint32_t last, int K, uint32_t cap, int tile) {
     bool ok = true;
     for (int k = 0; k < K && ok; k++) {
       for (int t = 0; t < kTypes && ok; t++) {
@@ -480,23 +508,7 @@ struct lbf_server {
         ok = hipGetLastError() == hipSuccess;
       }
     }
-    hipGraph_t g = nullptr;
-    const hipError_t e = hipStreamEndCapture(stream, &g);
-    if (!ok || e != hipSuccess || !g) {
-      if (g) (void)hipGraphDestroy(g);
-      return err(LBF_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
-    }
-    hipGraphExec_t x = nullptr;
-    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
-    (void)hipGraphDestroy(g);
-    if (ei != hipSuccess) return err(LBF_EDEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei));
-    graphs[key] = x;
-    {
-      std::lock_guard<std::mutex> lk(mu);
-      graphs_built++;
-    }
-    *out = x;
-    return 0;
+    return ok;
   }
 
   // Launch shape (workgroups, tile size index) for the next round of this mix.
@@ -697,12 +709,17 @@ struct lbf_server {
           if (slot_req[i]->remaining <= K) last |= 1u << slot_req[i]->type;
         }
       hipGraphExec_t g = nullptr;
-      if ((rc = get_graph(mask, last, K, grid, tile, &g))) {
+      if (!eager && (rc = get_graph(mask, last, K, grid, tile, &g))) {
         fail_all_resident(rc);
         continue;
       }
       hipError_t e = hipEventRecord(ev0, stream);
-      if (e == hipSuccess) e = hipGraphLaunch(g, stream);
+      if (e == hipSuccess) {
+        if (eager)
+          e = enqueue_round(mask, last, K, grid, tile) ? hipSuccess : hipErrorLaunchFailure;
+        else
+          e = hipGraphLaunch(g, stream);
+      }
       if (e == hipSuccess) e = hipEventRecord(ev1, stream);
       if (e == hipSuccess) e = hipEventSynchronize(ev1);
       if (e != hipSuccess) {
@@ -826,6 +843,10 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
   s->cfg_tile = tile_index_of(s->cfg.tile);
   if (s->cfg_tile < 0) return err(LBF_EINVAL, "tile %u: use 128, 256 or 512", s->cfg.tile);
   s->cur_tile = s->cfg.autotune ? kDefaultTileIdx : s->cfg_tile;
+  {
+    const char* env = getenv("LBF_SERVE_EAGER");
+    s->eager = env && env[0] == '1';
+  }
   if (hipGetDevice(&s->device) != hipSuccess) return err(LBF_EDEVICE, "hipGetDevice failed");
   if (!s->cfg.hbm_budget_bytes) {
     size_t free_b = 0, total_b = 0;

@@ -45,6 +45,30 @@ def test_served_case_equals_single_case_path(vs, side, prec, steps):
     assert np.array_equal(r0, r1) and np.array_equal(u0, u1)
 
 
+def test_eager_rounds_equal_graph_replays(monkeypatch):
+    """LBF_SERVE_EAGER=1 (read when a server is created) enqueues each round's
+    launches directly instead of replaying a captured graph: same kernels in
+    the same order, so the same bits, and no graph is built."""
+    specs = [(case(abi.D3Q19, 20, abi.F16_SHIFTED, seed=5), 29),
+             (case(abi.D2Q9, 36, abi.F32, seed=6), 41)]
+    print("[serve-eager] graph replays", flush=True)
+    with Server(max_steps_per_round=8) as srv:
+        ts = [srv.submit(sp, n, keep_pops=True) for sp, n in specs]
+        srv.drain(120)
+        replayed = [srv.result(t) for t in ts]
+    monkeypatch.setenv("LBF_SERVE_EAGER", "1")
+    print("[serve-eager] eager rounds", flush=True)
+    with Server(max_steps_per_round=8) as srv:
+        ts = [srv.submit(sp, n, keep_pops=True) for sp, n in specs]
+        srv.drain(120)
+        eager = [srv.result(t) for t in ts]
+        st = srv.stats()
+    assert st["graphs_built"] == 0 and st["rounds"] >= 1
+    for a, b in zip(replayed, eager):
+        for x, y in zip(a, b):
+            assert np.array_equal(x, y)
+
+
 def test_result_is_bitwise_the_same_alone_and_in_a_mix():
     target = case(abi.D3Q19, 24, abi.F32, seed=11, tau=0.7)
     steps = 45
