@@ -9,6 +9,14 @@ so factor = B / counter, applied to every other kernel of the same run.
     python -m lbflow_amd.profiling trace  <kernel_stats.csv> --cmd "..."
     python -m lbflow_amd.profiling traffic <fetch_counter_collection.csv> \
         <write_counter_collection.csv> --copy-bytes 2147483648 --cmd-fetch "..." --cmd-write "..."
+    python -m lbflow_amd.profiling run-geometries --steps 10      # the program to trace
+    python -m lbflow_amd.profiling counters <counter_collection.csv>... --cmd "..." [--cmd ...]
+
+`run-geometries` steps the 256^3 D3Q19 TRT porous channel and then the
+all-fluid periodic box, in FP16-shifted and then FP32 storage, through the
+in-place kernel; `counters` assigns each step launch to its geometry by
+dispatch order and tabulates the memory-pipeline counters per wavefront
+memory instruction.
 """
 import argparse
 import collections
@@ -111,6 +119,88 @@ def read_traffic(path):
     return res
 
 
+GEOMETRY_STORAGES = ("f16s", "f32")
+GEOMETRIES = ("porous", "all_fluid")
+
+
+def run_geometries(steps, n=256):
+    """Steps each geometry in each storage (order: storage, then porous before
+    all-fluid), one case at a time, on the in-place kernel."""
+    from . import abi, configs
+    from .engine import Case
+    prec = {"f16s": abi.F16_SHIFTED, "f32": abi.F32}
+    for st in GEOMETRY_STORAGES:
+        for geo in GEOMETRIES:
+            if geo == "porous":
+                sp = configs.porous_channel(n=n, precision=prec[st], streaming=abi.STREAM_AA)
+            else:
+                sp = configs.periodic_box(n=n, precision=prec[st], streaming=abi.STREAM_AA).replace(
+                    initial_velocity=(0.05, 0.0, 0.0))
+            c = Case(sp)
+            c.initialize_nodes()
+            c.main_steps(steps)
+            c.synchronize()
+            c.close()
+            print(f"[run-geometries] {st} {geo}: {steps} steps", flush=True)
+
+
+# memory-pipeline counters per wavefront memory instruction; the _sum counters
+# add up every instance (one TCP per CU, one TCC per L2 channel),
+# GRBM_GUI_ACTIVE counts the GPU's busy cycles over the dispatch
+def counters_summary(paths, commands):
+    per = collections.defaultdict(lambda: collections.defaultdict(list))
+    for path in paths:
+        disp = collections.defaultdict(dict)
+        names = {}
+        for r in csv.DictReader(open(path)):
+            d = int(r["Dispatch_Id"])
+            disp[d][r["Counter_Name"]] = disp[d].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+            names[d] = r["Kernel_Name"]
+        by_kernel = collections.defaultdict(list)
+        for d in sorted(disp):
+            m = re.search(r"momentum_step_aa<lbf::D3Q19, lbf::(\w+), 1, (true|false), (true|false)>",
+                          names[d])
+            if m and m.group(3) == "false":  # streaming launches (no moment store)
+                by_kernel[(_STORAGE.get(m.group(1)), m.group(2) == "true")].append(d)
+        for (st, odd), ds in by_kernel.items():
+            half = len(ds) // 2
+            for gi, geo in enumerate(GEOMETRIES):
+                part = ds[gi * half:(gi + 1) * half]
+                if not odd:
+                    part = part[1:]  # the first even launch reads natural slots
+                for d in part:
+                    for cn, v in disp[d].items():
+                        per[(st, geo, "odd" if odd else "even")][cn].append(v)
+    mean = {k: {cn: sum(v) / len(v) for cn, v in c.items()} for k, c in per.items()}
+    out = [f"command: {c}" for c in commands]
+    out += ["", "Per launch of the in-place step kernel (D3Q19 TRT, 256^3), means over the "
+            "streaming launches of each parity:", ""]
+    cols = [("VMEM rd/wave", lambda c: c["SQ_INSTS_VMEM_RD"] / c["SQ_WAVES"]),
+            ("VMEM wr/wave", lambda c: c["SQ_INSTS_VMEM_WR"] / c["SQ_WAVES"]),
+            ("TCP acc/instr", lambda c: c["TCP_TOTAL_CACHE_ACCESSES_sum"] /
+             (c["SQ_INSTS_VMEM_RD"] + c["SQ_INSTS_VMEM_WR"])),
+            ("L2 rd req/ld", lambda c: c["TCP_TCC_READ_REQ_sum"] / c["SQ_INSTS_VMEM_RD"]),
+            ("L2 wr req/st", lambda c: c["TCP_TCC_WRITE_REQ_sum"] / c["SQ_INSTS_VMEM_WR"]),
+            ("L2 hit %", lambda c: 100 * c["TCC_HIT_sum"] / (c["TCC_HIT_sum"] + c["TCC_MISS_sum"])),
+            ("busy Mcyc", lambda c: c["GRBM_GUI_ACTIVE"] / 1e6)]
+    out.append(f"{'storage':8s} {'geometry':10s} {'step':5s} " +
+               " ".join(f"{name:>13s}" for name, _ in cols))
+    for st in GEOMETRY_STORAGES:
+        for geo in GEOMETRIES:
+            for par in ("even", "odd"):
+                c = mean.get((st, geo, par))
+                if not c:
+                    continue
+                vals = []
+                for _, f in cols:
+                    try:
+                        vals.append(f"{f(c):13.3f}")
+                    except (KeyError, ZeroDivisionError):
+                        vals.append(f"{'-':>13s}")
+                out.append(f"{st:8s} {geo:10s} {par:5s} " + " ".join(vals))
+    return "\n".join(out) + "\n"
+
+
 def main():
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -124,9 +214,18 @@ def main():
     m.add_argument("--nodes", type=int, default=256 ** 3)
     m.add_argument("--cmd-fetch", required=True)
     m.add_argument("--cmd-write", required=True)
+    g = sub.add_parser("run-geometries")
+    g.add_argument("--steps", type=int, default=10)
+    c = sub.add_parser("counters")
+    c.add_argument("csv", nargs="+")
+    c.add_argument("--cmd", action="append", default=[])
     a = ap.parse_args()
     if a.what == "trace":
         print(trace_summary(a.stats_csv, a.cmd), end="")
+    elif a.what == "run-geometries":
+        run_geometries(a.steps)
+    elif a.what == "counters":
+        print(counters_summary(a.csv, a.cmd), end="")
     else:
         print(traffic_summary(a.fetch_csv, a.write_csv, a.copy_bytes, a.cmd_fetch, a.cmd_write,
                               a.nodes), end="")
