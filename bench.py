@@ -372,9 +372,22 @@ def serving_leg(sample=48, plain_sample=12, two_array_sample=6, seed=5, cpu=True
     t0 = time.perf_counter()
     with Server() as srv:
         tickets = [srv.submit(sp, n) for sp, n in specs]
-        for t in tickets:
-            srv.wait(t, 3600)
-            srv.result(t)
+        # each result is fetched as its case completes (finished results would
+        # otherwise wait in host memory), with a progress line every minute
+        pending, last = set(tickets), t0
+        while pending:
+            for t in list(pending):
+                if srv.info(t).state in (abi.REQ_DONE, abi.REQ_FAILED):
+                    srv.result(t)
+                    pending.discard(t)
+            if pending:
+                time.sleep(0.02)
+            if time.perf_counter() - last > 60:
+                last = time.perf_counter()
+                print(f"[serve] {len(tickets) - len(pending)} of {len(tickets)} cases done after "
+                      f"{last - t0:.0f} s", file=sys.stderr, flush=True)
+            if time.perf_counter() - t0 > 3600:
+                raise RuntimeError("serving sample did not finish within an hour")
         st = srv.stats()
     wall = time.perf_counter() - t0
     served_mix_s = wall * mix_updates / sample_updates
