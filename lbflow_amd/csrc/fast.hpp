@@ -120,6 +120,18 @@ __device__ __forceinline__ void meq_shifted(float G, float B, float sc, const fl
   }
 }
 
+// A device-memory pointer as a global-address-space one: pointers read from
+// memory (the serving tier's slot table) are otherwise generic and loaded
+// through flat instructions; kernel-argument pointers are global already.
+template <class T>
+__device__ __forceinline__ const __attribute__((address_space(1))) T* global_ptr(const T* p) {
+  return (const __attribute__((address_space(1))) T*)p;
+}
+template <class T>
+__device__ __forceinline__ __attribute__((address_space(1))) T* global_ptr(T* p) {
+  return (__attribute__((address_space(1))) T*)p;
+}
+
 // ------------------------------------------------------------ buffer I/O
 template <class S>
 __device__ __forceinline__ S bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
@@ -499,10 +511,10 @@ __device__ __forceinline__ void aa_node_cls(const KArgs& a, uint32_t n, uint32_t
       __builtin_amdgcn_make_buffer_rsrc(a.pop[0], (short)0, (int)0xffffffff, 0x00020000);
   const uint32_t pb = a.plane_bytes;
   const int par = (int)(step & 1);
-  T* rho_c = (T*)a.rho[par];
-  T* u_c = (T*)a.u[par];
-  const T* rho_p = (const T*)a.rho[par ^ 1];
-  const T* u_p = (const T*)a.u[par ^ 1];
+  __attribute__((address_space(1))) T* rho_c = global_ptr((T*)a.rho[par]);
+  __attribute__((address_space(1))) T* u_c = global_ptr((T*)a.u[par]);
+  const __attribute__((address_space(1))) T* rho_p = global_ptr((const T*)a.rho[par ^ 1]);
+  const __attribute__((address_space(1))) T* u_p = global_ptr((const T*)a.u[par ^ 1]);
   const bool store_m = MOM && (a.flags & (kFlagStoreMoments | kFlagResiduals)) != 0;
   const bool resid = MOM && (a.flags & kFlagResiduals) != 0;
   const T omega = T(a.omega), omega_p = T(a.omega_p), wp = T(a.wp), wm = T(a.wm),
@@ -573,14 +585,22 @@ __device__ __forceinline__ void aa_node_cls(const KArgs& a, uint32_t n, uint32_t
         });
         u[ax] = s * ju;
       });
+      // the uniform force and the per-node field are read in separate branches:
+      // a select between their addresses would load the uniform value per lane
       T Fv[3] = {T(0), T(0), T(0)};
-      if (a.force_kind) {
+      if (a.force_kind == 1) {
+        sfor<D>([&](auto A) { Fv[decltype(A)::value] = T(a.F[decltype(A)::value]); });
+      } else if (a.force_kind) {
         sfor<D>([&](auto A) {
           constexpr int ax = decltype(A)::value;
-          Fv[ax] = a.force_kind == 1 ? T(a.F[ax]) : T(a.src_field[(size_t)ax * ld + n]);
-          u[ax] += T(0.5) * Fv[ax] * irho;
+          Fv[ax] = T(global_ptr(a.src_field)[(size_t)ax * ld + n]);
         });
       }
+      if (a.force_kind)
+        sfor<D>([&](auto A) {
+          constexpr int ax = decltype(A)::value;
+          u[ax] += T(0.5) * Fv[ax] * irho;
+        });
       store_moments(rho, u);
       // ---- collision. f_eq,i = w_i (A + B t_i), t_i = 3 u.c_i + 4.5 (u.c_i)^2 - 1.5 u.u,
       // with A = B = rho for plain storage and A = scale * drho, B = scale * rho in
@@ -715,7 +735,7 @@ __device__ __forceinline__ void aa_node_cls(const KArgs& a, uint32_t n, uint32_t
 
 // class word of node n of a case (kSolidBit past its last node)
 __device__ __forceinline__ uint32_t node_class(const KArgs& a, uint32_t n) {
-  return n < (uint32_t)a.N ? a.cls[n] : kSolidBit;
+  return n < (uint32_t)a.N ? global_ptr(a.cls)[n] : kSolidBit;
 }
 
 template <class L, class St, int OP, bool ODD, bool MOM>
