@@ -215,6 +215,9 @@ struct lbf_case {
   int ns = 0;
   bool aa = false;       // in-place AA scheme (one population array)
   uint64_t host_step = 0;  // AA: steps enqueued (device count, resynchronised on reads)
+  // AA: streaming steps enqueued since the device step counter last advanced
+  // (their launches carry it as step_offset; flush_steps advances the counter)
+  uint32_t pending_steps = 0;
   bool aa_fresh = false;   // AA: array holds post-stream f (after init / set_pops)
   int xcd_group = 0;       // AA: runs of consecutive blocks per XCD (0: round-robin)
   bool may_have_stopped = false;  // steps were enqueued with the stop rule on
@@ -791,11 +794,29 @@ uint32_t residual_mask(lbf_case* c) {
   return m;
 }
 
+// Advances the device step counter by the in-place steps enqueued without
+// their own advancing launch.
+int flush_steps(lbf_case* c) {
+  if (c->pending_steps) {
+    HIPCHK(launch_ctrl_set(c->ctrl, kCtrlAddSteps, c->pending_steps, c->stream));
+    c->pending_steps = 0;
+  }
+  return 0;
+}
+
 // Enqueues one time step: momentum launch, then one launch per scalar; the
-// last launch advances the step.
+// last launch advances the step. An in-place step without residuals or the
+// stop rule has nothing for that launch to reduce: it reads the counter plus
+// the steps enqueued since it last advanced (step_offset), and flush_steps
+// advances the counter once for the batch.
 int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
   const bool resid = flags & LBF_STEP_RESIDUALS;
   const bool stop = flags & LBF_STEP_STOP_ON_CONVERGENCE;
+  const bool defer = c->aa && !c->frozen && c->ns == 0 && !resid && !stop;
+  if (!defer) {
+    int rc = flush_steps(c);
+    if (rc) return rc;
+  }
   uint32_t kf = 0;
   if (resid) kf |= kFlagResiduals;
   if (stop) kf |= kFlagStopCheck;
@@ -806,6 +827,7 @@ int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
     a.flags = kf & ~kFlagStopCheck;
     if (resid || last_of_batch || c->needs_moments) a.flags |= kFlagStoreMoments;
     a.res_mask = mask;
+    a.step_offset = defer ? c->pending_steps : 0;
     if (c->aa) {
       HIPCHK(c->L.aa_step(a, c->grid, c->stream));
     } else {
@@ -825,7 +847,9 @@ int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
     a.res_mask = mask;
     HIPCHK(c->L.sca_step(a, c->grid, c->stream));
   }
-  {
+  if (defer) {
+    c->pending_steps++;
+  } else {
     // one-workgroup launch: residual reduction, stop rule, step counter
     KArgs a = momentum_args(c);
     a.flags = kf;
@@ -851,6 +875,8 @@ int step_graph(lbf_case* c, uint32_t flags, bool every_last, hipGraphExec_t* out
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = 0;
   for (uint32_t k = 0; k < kGraphSteps && !rc; k++) rc = enqueue_step(c, flags, every_last);
+  if (!rc) rc = flush_steps(c);  // the graph advances the counter for its own steps
+  c->pending_steps = 0;
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(c->stream, &g);
   c->host_step = step0;  // capture enqueued nothing
@@ -876,7 +902,8 @@ int enqueue_steps(lbf_case* c, uint64_t nsteps, uint32_t flags, bool every_last)
     if (c->graphs_on && !c->aa_fresh && (c->host_step & 1) == 0 &&
         (every_last ? left >= kGraphSteps : left > kGraphSteps)) {
       hipGraphExec_t g = nullptr;
-      int rc = step_graph(c, flags, every_last, &g);
+      int rc = flush_steps(c);  // the graph's launches count from the device counter
+      if (!rc) rc = step_graph(c, flags, every_last, &g);
       if (rc) return rc;
       HIPCHK(hipGraphLaunch(g, c->stream));
       c->host_step += kGraphSteps;
@@ -887,7 +914,7 @@ int enqueue_steps(lbf_case* c, uint64_t nsteps, uint32_t flags, bool every_last)
       k++;
     }
   }
-  return 0;
+  return flush_steps(c);
 }
 
 }  // namespace

@@ -65,6 +65,7 @@ enum CtrlOp : int {
   kCtrlFresh = 0,     // value: field mask to mark fresh
   kCtrlResetStop = 1,
   kCtrlMaxIter = 2,   // value: max_iter
+  kCtrlAddSteps = 3,  // value: steps run since the counter last advanced (unless stopped)
 };
 
 }  // namespace lbf

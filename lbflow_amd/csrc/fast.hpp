@@ -760,7 +760,7 @@ __global__ void __launch_bounds__(kBlock)
     momentum_step_aa(KArgs a) {
   Ctrl* ctrl = a.ctrl;
   if (ctrl->stopped) return;
-  const unsigned long long step = ctrl->step;
+  const unsigned long long step = ctrl->step + a.step_offset;
   double acc[5] = {0, 0, 0, 0, 0};
   // one node per thread (the host launches ceil(N / kBlock) workgroups),
   // node blocks in XCD runs (block_index)

@@ -21,6 +21,12 @@ __global__ void ctrl_set_kernel(Ctrl* ctrl, int what, long long value) {
     case kCtrlMaxIter:
       ctrl->max_iter = (unsigned long long)value;
       break;
+    case kCtrlAddSteps:
+      if (!ctrl->stopped) {
+        ctrl->step += (unsigned long long)value;
+        for (int f = 0; f < 5; f++) ctrl->fresh[f] = 0;
+      }
+      break;
   }
 }
 
