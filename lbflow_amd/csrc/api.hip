@@ -215,7 +215,7 @@ struct lbf_case {
   int ns = 0;
   bool aa = false;       // in-place AA scheme (one population array)
   uint64_t host_step = 0;  // AA: steps enqueued (device count, resynchronised on reads)
-  // AA: streaming steps enqueued since the device step counter last advanced
+  // steps enqueued since the device step counter last advanced
   // (their launches carry it as step_offset; flush_steps advances the counter)
   uint32_t pending_steps = 0;
   bool aa_fresh = false;   // AA: array holds post-stream f (after init / set_pops)
@@ -805,14 +805,14 @@ int flush_steps(lbf_case* c) {
 }
 
 // Enqueues one time step: momentum launch, then one launch per scalar; the
-// last launch advances the step. An in-place step without residuals or the
-// stop rule has nothing for that launch to reduce: it reads the counter plus
+// last launch advances the step. A step without residuals or the stop rule
+// has nothing for that launch to reduce: its launches read the counter plus
 // the steps enqueued since it last advanced (step_offset), and flush_steps
 // advances the counter once for the batch.
 int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
   const bool resid = flags & LBF_STEP_RESIDUALS;
   const bool stop = flags & LBF_STEP_STOP_ON_CONVERGENCE;
-  const bool defer = c->aa && !c->frozen && c->ns == 0 && !resid && !stop;
+  const bool defer = !resid && !stop;
   if (!defer) {
     int rc = flush_steps(c);
     if (rc) return rc;
@@ -845,6 +845,7 @@ int enqueue_step(lbf_case* c, uint32_t flags, bool last_of_batch) {
     a.mode = kModePull;
     a.flags = kf & ~kFlagStopCheck;
     a.res_mask = mask;
+    a.step_offset = defer ? c->pending_steps : 0;
     HIPCHK(c->L.sca_step(a, c->grid, c->stream));
   }
   if (defer) {

@@ -66,7 +66,7 @@ struct KArgs {
   int faces_noslip_only; // every non-periodic face is no-slip (AA: nothing to apply)
   int fresh;             // AA: the array holds post-stream f in natural slots
   int aa_odd;            // AA: parity of the step being launched
-  uint32_t step_offset;  // AA: steps enqueued since the device step counter last advanced
+  uint32_t step_offset;  // steps enqueued since the device step counter last advanced
   uint32_t plane_bytes;  // AA: bytes per population plane (array < 4 GiB)
   int xcd_group;         // runs of node blocks per XCD (0 = round-robin order)
   void* pop[2];          // ping-pong population buffers (two-array scheme)

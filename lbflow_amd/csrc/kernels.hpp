@@ -188,7 +188,7 @@ __device__ __forceinline__ void apply_momentum_faces(const KArgs& a, uint32_t on
         // 1.5 u(x) - 0.5 u(x + n_in), src/momentum/bc.rs:93-101
         constexpr int nd = Dv::normal(F);
         const long long nb = n + g.off(k::cx(nd), k::cy(nd), k::cz(nd));
-        const int pv = (int)((a.ctrl->step + 1) & 1);  // previous step's moments
+        const int pv = (int)((a.ctrl->step + a.step_offset + 1) & 1);  // previous step's moments
         const T* up = (const T*)a.u[pv];
         sfor<D>([&](auto A) {
           constexpr int ax = decltype(A)::value;
@@ -285,9 +285,11 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
   constexpr int Q = L::Q, D = L::D;
   Ctrl* ctrl = a.ctrl;
   if (a.mode != kModeStreamOnly && ctrl->stopped) return;
-  const unsigned long long step = ctrl->step;
+  // a launch of a batch whose counter has not advanced yet counts on from it;
+  // only the batch's first step can read post-stream f
+  const unsigned long long step = ctrl->step + a.step_offset;
   const int par = (int)(step & 1);
-  const bool fresh = ctrl->fresh[0] != 0;
+  const bool fresh = a.step_offset == 0 && ctrl->fresh[0] != 0;
   T* phi = (T*)a.phi;
   const S* __restrict__ src = (const S*)a.pop[par];
   S* __restrict__ dst = (S*)a.pop[par ^ 1];
@@ -540,9 +542,9 @@ __global__ void __launch_bounds__(kBlock) scalar_step(KArgs a) {
   constexpr int Q = L::Q, D = L::D;
   Ctrl* ctrl = a.ctrl;
   if (a.mode == kModePull && ctrl->stopped) return;
-  const unsigned long long step = ctrl->step;
+  const unsigned long long step = ctrl->step + a.step_offset;
   const int par = (int)(step & 1);
-  const bool fresh = ctrl->fresh[a.field] != 0;
+  const bool fresh = a.step_offset == 0 && ctrl->fresh[a.field] != 0;
   const T* __restrict__ src = (const T*)a.pop[par];
   T* __restrict__ dst = (T*)a.pop[par ^ 1];
   T* c_c = (T*)a.c[par];
