@@ -766,15 +766,7 @@ __global__ void __launch_bounds__(kBlock)
   // node blocks in XCD runs (block_index)
   const uint32_t n = block_index(a) * kBlock + threadIdx.x;
   aa_node<L, St, OP, ODD, MOM>(a, n, step, a.fresh != 0, acc);
-  if (a.flags & kFlagResiduals) {
-    __shared__ double lds[kBlock / 64];
-    for (int kk = 0; kk < 4; kk++) {
-      const double s = block_sum(acc[kk], lds);
-      if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + kk] = s;
-    }
-    const double s = block_sum(acc[4], lds);
-    if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + 8] = s;
-  }
+  if (a.flags & kFlagResiduals) store_residual_partials(acc, a.partials + (long long)blockIdx.x * kMaxCols);
 }
 
 

@@ -109,6 +109,26 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Residual partials of one workgroup: columns 0 (rho), 1..3 (u) and 8
+// (non-finite count) of its row, the five sums taken together (one wave
+// reduction each, one barrier) in block_sum's order, so the bits are the
+// same as five block_sum calls.
+__device__ __forceinline__ void store_residual_partials(const double* acc, double* row) {
+  __shared__ double lds[5][kBlock / 64];
+  double v[5];
+  for (int k = 0; k < 5; k++) v[k] = wave_sum(acc[k]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int k = 0; k < 5; k++) lds[k][wid] = v[k];
+  __syncthreads();
+  if (threadIdx.x < 5) {
+    const int k = threadIdx.x;
+    double r = lds[k][0];
+    for (int w = 1; w < kBlock / 64; w++) r += lds[k][w];
+    row[k < 4 ? k : 8] = r;
+  }
+}
+
 // deterministic workgroup sum; result valid in thread 0
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
@@ -422,15 +442,7 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
   }
   if (a.mode != kModePull) return;
   // residual partials: columns 0 rho, 1..3 u, 8 non-finite count
-  if (a.flags & kFlagResiduals) {
-    __shared__ double lds[kBlock / 64];
-    for (int kk = 0; kk < 4; kk++) {
-      const double s = block_sum(acc[kk], lds);
-      if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + kk] = s;
-    }
-    const double s = block_sum(acc[4], lds);
-    if (threadIdx.x == 0) a.partials[(long long)blockIdx.x * kMaxCols + 8] = s;
-  }
+  if (a.flags & kFlagResiduals) store_residual_partials(acc, a.partials + (long long)blockIdx.x * kMaxCols);
 }
 
 // The parity kernel's arithmetic after streaming, for the in-place kernel

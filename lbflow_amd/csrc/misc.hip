@@ -49,17 +49,30 @@ __global__ void __launch_bounds__(kBlock) advance_kernel(KArgs a, long long nblo
 // 65,536 rows at 256^3).
 __global__ void __launch_bounds__(kBlock) fold_partials(double* p, long long nblocks,
                                                         uint32_t mask) {
-  __shared__ double lds[kBlock / 64];
+  // every column in one pass over the rows (each thread sums its rows in
+  // order, then the workgroup sums in block_sum's order: the same bits as
+  // one block_sum per column)
+  __shared__ double lds[kMaxCols][kBlock / 64];
   const long long b0 = (long long)blockIdx.x * kResidualFold;
+  double s[kMaxCols];
+  for (int k = 0; k < kMaxCols; k++) s[k] = 0.0;
+  for (int i = threadIdx.x; i < kResidualFold; i += kBlock) {
+    const long long b = b0 + i;
+    if (b < nblocks)
+      for (int k = 0; k < kMaxCols; k++)
+        if (mask >> k & 1u) s[k] += p[b * kMaxCols + k];
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int k = 0; k < kMaxCols; k++) {
-    if (!(mask >> k & 1u)) continue;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < kResidualFold; i += kBlock) {
-      const long long b = b0 + i;
-      if (b < nblocks) s += p[b * kMaxCols + k];
-    }
-    s = block_sum(s, lds);
-    if (threadIdx.x == 0) p[(nblocks + blockIdx.x) * kMaxCols + k] = s;
+    const double v = wave_sum(s[k]);
+    if (lane == 0) lds[k][wid] = v;
+  }
+  __syncthreads();
+  const int k = threadIdx.x;
+  if (k < kMaxCols && (mask >> k & 1u)) {
+    double r = lds[k][0];
+    for (int w = 1; w < kBlock / 64; w++) r += lds[k][w];
+    p[(nblocks + blockIdx.x) * kMaxCols + k] = r;
   }
 }
 
