@@ -72,10 +72,14 @@ constexpr int kTypes = 16;
 constexpr int kCounters = 2 * kTypes + 1;
 constexpr int kLastFlags = 2 * kTypes;
 constexpr int kHelpers = 8;  // threads that set up and release cases
-// caps on the workgroups of one batched launch the autotuner tries; a launch
-// takes min(cap, its tile count rounded up to a power of two), so the largest
-// cap runs one tile per workgroup
-constexpr uint32_t kGridCandidates[] = {2048, 8192, 32768, 1u << 22};
+// caps on the workgroups of one batched launch the autotuner tries, the
+// starting shape first; a launch takes min(cap, its tile count rounded up to a
+// power of two), so the first cap runs one tile per workgroup
+constexpr uint32_t kGridCandidates[] = {1u << 22, 32768, 8192, 2048};
+// a measured shape replaces the best one only when it is this much faster:
+// the rounds that measure successive shapes also differ in which cases they
+// run, so small differences are noise
+constexpr double kTuneMargin = 0.97;
 // workgroups of a last-step launch at most (it usually returns at once)
 constexpr uint32_t kLastGridMax = 2048;
 // distinct graphs kept; past this the cache is emptied
@@ -130,7 +134,7 @@ struct Tune {
   int phase = 0;  // 0: grid candidates, 1: tile candidates
   int next = 0;
   double best = 1e300;
-  uint32_t best_grid = 2048;
+  uint32_t best_grid = kGridCandidates[0];
   int best_tile = kDefaultTileIdx;
   bool done = false;
 };
@@ -547,11 +551,14 @@ struct lbf_server {
     }
   }
 
-  void record_tuning(uint32_t mask, uint32_t grid, int tile, double seconds_per_update) {
+  // seconds_per_byte: the round's time over its algorithmic bytes (2 Q s per
+  // node update), which weighs 16-bit and 32-bit, D2Q9 and D3Q19 cases by the
+  // traffic they need, so rounds of different composition compare better
+  void record_tuning(uint32_t mask, uint32_t grid, int tile, double seconds_per_byte) {
     Tune& t = tune[mask];
     if (t.done) return;
-    if (seconds_per_update < t.best) {
-      t.best = seconds_per_update;
+    if (t.best == 1e300 || seconds_per_byte < kTuneMargin * t.best) {
+      t.best = seconds_per_byte;
       t.best_grid = grid;
       t.best_tile = tile;
     }
@@ -701,12 +708,15 @@ struct lbf_server {
       // rest of it (its remaining count is 0 on the device)
       int K = 1;
       while (2 * K <= (long long)cfg.max_steps_per_round && K < max_rem) K *= 2;
-      double work = 0;  // node updates this replay performs
+      double work = 0;  // algorithmic bytes this replay moves (2 Q s per node update)
       uint32_t last = 0;  // types with a case whose last step falls in this replay
       for (int i = 0; i < kServeMaxSlots; i++)
         if (slot_req[i]) {
-          work += (double)slot_req[i]->nodes * (double)std::min<long long>(K, slot_req[i]->remaining);
-          if (slot_req[i]->remaining <= K) last |= 1u << slot_req[i]->type;
+          const Request* r = slot_req[i];
+          const int prec = LBF_F32 + (r->type / 2) % 4;
+          const double s = (prec == LBF_F32 || prec == LBF_F32_SHIFTED) ? 4.0 : 2.0;
+          work += 2.0 * r->q * s * (double)r->nodes * (double)std::min<long long>(K, r->remaining);
+          if (r->remaining <= K) last |= 1u << r->type;
         }
       hipGraphExec_t g = nullptr;
       if (!eager && (rc = get_graph(mask, last, K, grid, tile, &g))) {
@@ -838,7 +848,7 @@ int lbf_server_create(const lbf_server_config* cfg, lbf_server** out) {
   if (!s->cfg.max_slots || s->cfg.max_slots > (uint32_t)kServeMaxSlots) s->cfg.max_slots = kServeMaxSlots;
   if (!s->cfg.max_steps_per_round) s->cfg.max_steps_per_round = 64;
   s->cfg.max_steps_per_round = std::min<uint32_t>(s->cfg.max_steps_per_round, 256);
-  if (!s->cfg.grid) s->cfg.grid = kGridCandidates[kNumCandidates - 1];
+  if (!s->cfg.grid) s->cfg.grid = kGridCandidates[0];
   if (!s->cfg.tile) s->cfg.tile = (uint32_t)kTileSizes[kDefaultTileIdx];
   s->cfg_tile = tile_index_of(s->cfg.tile);
   if (s->cfg_tile < 0) return err(LBF_EINVAL, "tile %u: use 128, 256 or 512", s->cfg.tile);
