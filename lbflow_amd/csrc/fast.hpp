@@ -59,6 +59,11 @@
 #ifndef LBF_WAVES_32_ODD
 #define LBF_WAVES_32_ODD 6
 #endif
+// waves per SIMD the moment-storing variant (last step of a batch, residual
+// steps) is compiled for at least
+#ifndef LBF_WAVES_MOM
+#define LBF_WAVES_MOM 2
+#endif
 
 namespace lbf {
 
@@ -750,7 +755,7 @@ __device__ __forceinline__ void aa_node(const KArgs& a, uint32_t n, unsigned lon
 // compiled for full occupancy.
 template <class L, class St, int OP, bool ODD, bool MOM>
 __global__ void __launch_bounds__(kBlock)
-    __attribute__((amdgpu_waves_per_eu(MOM ? 2
+    __attribute__((amdgpu_waves_per_eu(MOM ? LBF_WAVES_MOM
                                        : (sizeof(typename St::T) == 8 && (L::Q > 19 || L::Q == 15)) ? 2
                                        : (sizeof(typename St::T) == 8 && OP == 0) ? 3
                                        : (sizeof(typename St::T) == 8 || L::Q > 19) ? 4
