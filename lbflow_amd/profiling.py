@@ -4,7 +4,8 @@ Turns `--kernel-trace --stats` and `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE`
 results into short text tables, and derives the HBM bytes each step launch
 moves. The memory counters are calibrated against the plain device copy
 (`lbfx_copy`) that bench.py runs first: a copy of B bytes reads B and writes B,
-so factor = B / counter, applied to every other kernel of the same run.
+so factor = B / counter (over the B-byte copies only, not bench.py's shorter
+warm-up copies), applied to every other kernel of the same run.
 
     python -m lbflow_amd.profiling trace  <kernel_stats.csv> --cmd "..."
     python -m lbflow_amd.profiling traffic <fetch_counter_collection.csv> \
@@ -73,8 +74,16 @@ def traffic(fetch_csv, write_csv, copy_bytes):
     copy = [k for k in f if "copy_kernel" in k]
     if not copy:
         raise ValueError("no copy_kernel dispatch to calibrate against")
-    ff = copy_bytes / (sum(f[copy[0]]) / len(f[copy[0]]))
-    fw = copy_bytes / (sum(w[copy[0]]) / len(w[copy[0]]))
+
+    def largest(vals):
+        # the copy_bytes copies of bench.py's bandwidth probe; its shorter
+        # warm-up copies before each leg read and write less
+        top = max(vals)
+        big = [v for v in vals if v >= 0.9 * top]
+        return sum(big) / len(big)
+
+    ff = copy_bytes / largest(f[copy[0]])
+    fw = copy_bytes / largest(w[copy[0]])
     per = {}
     for k in f:
         if k in w:
@@ -97,7 +106,8 @@ def traffic_summary(fetch_csv, write_csv, copy_bytes, cmd_fetch, cmd_write, node
         out.append(f"{short_name(k):88s} {cnt:3d} {rb / 1e9:8.3f} {wb / 1e9:8.3f} "
                    f"{(alg or 0) / 1e9:7.3f} {ratio}")
         # streaming steps (no moment store) are the launches a run is made of
-        if v and ("true, true>" not in k):
+        mom = re.search(r"momentum_step_aa<[^>]*, (true|false), (true|false)>", k)
+        if v and not (mom and mom.group(2) == "true"):
             by_variant[v].append(rb + wb)
     out.append("")
     out.append("HBM bytes per step launch (mean of the even and odd streaming kernels):")
