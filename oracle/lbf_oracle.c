@@ -1281,6 +1281,9 @@ static int one_step(lbo_case* L, unsigned flags, unsigned long long max_iter) {
 }
 
 int lbo_main_steps(lbo_case* L, unsigned nsteps, unsigned flags) {
+  // every call starts unstopped, as lbf_main_steps does: a stop ends the
+  // call's loop, not the case (src/momentum.rs:273-292 only ends the loop)
+  L->stopped = 0;
   for (unsigned k = 0; k < nsteps; k++) {
     if (L->stopped) break;
     int rc = one_step(L, flags, ~0ull);
