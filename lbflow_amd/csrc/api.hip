@@ -214,7 +214,7 @@ struct lbf_case {
   ScalarDev sc[LBF_MAX_SCALARS];
   int ns = 0;
   bool aa = false;       // in-place AA scheme (one population array)
-  uint64_t host_step = 0;  // AA: steps enqueued (device count, resynchronised on reads)
+  uint64_t host_step = 0;  // steps enqueued (device count, resynchronised on reads)
   // steps enqueued since the device step counter last advanced
   // (their launches carry it as step_offset; flush_steps advances the counter)
   uint32_t pending_steps = 0;
@@ -794,8 +794,8 @@ uint32_t residual_mask(lbf_case* c) {
   return m;
 }
 
-// Advances the device step counter by the in-place steps enqueued without
-// their own advancing launch.
+// Advances the device step counter by the steps enqueued without their own
+// advancing launch.
 int flush_steps(lbf_case* c) {
   if (c->pending_steps) {
     HIPCHK(launch_ctrl_set(c->ctrl, kCtrlAddSteps, c->pending_steps, c->stream));

@@ -5,7 +5,8 @@
 //  - the worker thread admits requests while the HBM budget and slots allow
 //    (first in first out, with backfill that never eats into what the oldest
 //    waiting request needs), activates prepared cases into slots, and runs
-//    rounds: one graph replay of K steps of every resident case, then retires
+//    rounds: one graph replay of K steps of every resident case (with
+//    LBF_SERVE_EAGER=1 the same launches enqueued directly), then retires
 //    the cases that finished;
 //  - eight helper threads create and initialise admitted cases (uploads,
 //    node classes, f = f_eq) and download and free finished ones, release
@@ -733,7 +734,8 @@ struct lbf_server {
       if (e == hipSuccess) e = hipEventRecord(ev1, stream);
       if (e == hipSuccess) e = hipEventSynchronize(ev1);
       if (e != hipSuccess) {
-        fail_all_resident(err(LBF_EDEVICE, "graph replay: %s", hipGetErrorString(e)));
+        fail_all_resident(err(LBF_EDEVICE, "%s: %s", eager ? "round launches" : "graph replay",
+                              hipGetErrorString(e)));
         continue;
       }
       float ms = 0.f;
