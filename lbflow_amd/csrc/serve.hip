@@ -131,14 +131,26 @@ struct Request {
 // Launch shape search for one mix of case types: first the workgroup counts at
 // the default tile size, then the other tile sizes at the best count; each
 // candidate is one round, scored as seconds per node update.
+// Launch-shape search for one mix of case types: each challenger shape (the
+// other workgroup caps at the best tile size, then the other tile sizes at
+// the best cap) runs in the round after one of the incumbent shape, and the
+// two are compared only when both rounds ran the same resident cases for the
+// same number of steps (a round of other cases measures the cases, not the
+// shape).
 struct Tune {
-  int phase = 0;  // 0: grid candidates, 1: tile candidates
-  int next = 0;
-  double best = 1e300;
+  int phase = 0;  // 0: workgroup caps, 1: tile sizes
+  int next = 1;   // challenger index in the phase (cap index; tile: 0..1 among the others)
   uint32_t best_grid = kGridCandidates[0];
   int best_tile = kDefaultTileIdx;
+  bool have_base = false;        // the previous round measured the incumbent
+  double base_score = 0;         // its seconds per algorithmic byte
+  uint64_t base_version = 0;     // resident-set version and steps of that round
+  int base_k = 0;
+  int tries = 0;                 // unpaired attempts of the current challenger
   bool done = false;
 };
+// unpaired attempts after which a challenger is passed over
+constexpr int kTuneTries = 4;
 
 // Finished searches, shared by every server of the process: (device, mix of
 // case types) -> (workgroups, tile size index).
@@ -206,6 +218,7 @@ struct lbf_server {
   // replaying captured graphs (same kernels, same order; for tracing tools)
   bool eager = false;
   std::map<uint32_t, Tune> tune;
+  uint64_t mix_version = 0;  // changes whenever a case joins or leaves the slots
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
 
   double now() const {
@@ -332,6 +345,7 @@ struct lbf_server {
     s.step = info.step;
     s.type_bit = 1u << r->type;
     dirty = true;
+    mix_version++;
     return 0;
   }
 
@@ -517,8 +531,11 @@ struct lbf_server {
   }
 
   // Launch shape (workgroups, tile size index) for the next round of this mix.
-  void choose_shape(uint32_t mask, bool* measuring, uint32_t* grid, int* tile) {
+  // Shape of the next round; *measuring: the round is timed for the search,
+  // *base: it runs the incumbent shape (otherwise the current challenger).
+  void choose_shape(uint32_t mask, bool* measuring, bool* base, uint32_t* grid, int* tile) {
     *measuring = false;
+    *base = false;
     if (!cfg.autotune) {
       *grid = cfg.grid;
       *tile = cfg_tile;
@@ -543,31 +560,56 @@ struct lbf_server {
       return;
     }
     *measuring = true;
-    if (t.phase == 0) {
+    if (!t.have_base) {
+      *base = true;
+      *grid = t.best_grid;
+      *tile = t.best_tile;
+    } else if (t.phase == 0) {
       *grid = kGridCandidates[t.next];
-      *tile = kDefaultTileIdx;
+      *tile = t.best_tile;
     } else {
       *grid = t.best_grid;
-      *tile = t.next < kDefaultTileIdx ? t.next : t.next + 1;  // the other tile sizes
+      *tile = t.next < t.best_tile ? t.next : t.next + 1;  // the other tile sizes
     }
   }
 
   // seconds_per_byte: the round's time over its algorithmic bytes (2 Q s per
-  // node update), which weighs 16-bit and 32-bit, D2Q9 and D3Q19 cases by the
-  // traffic they need, so rounds of different composition compare better
-  void record_tuning(uint32_t mask, uint32_t grid, int tile, double seconds_per_byte) {
+  // node update); version / k: the resident set and steps the round ran.
+  void record_tuning(uint32_t mask, bool base, uint32_t grid, int tile, double seconds_per_byte,
+                     uint64_t version, int k) {
     Tune& t = tune[mask];
     if (t.done) return;
-    if (t.best == 1e300 || seconds_per_byte < kTuneMargin * t.best) {
-      t.best = seconds_per_byte;
-      t.best_grid = grid;
-      t.best_tile = tile;
-    }
-    if (t.phase == 0) {
-      if (++t.next >= kNumCandidates) t.phase = 1, t.next = 0;
+    if (base) {
+      t.have_base = true;
+      t.base_score = seconds_per_byte;
+      t.base_version = version;
+      t.base_k = k;
       return;
     }
-    if (++t.next < kNumTileSizes - 1) return;
+    t.have_base = false;
+    if (version == t.base_version && k == t.base_k) {
+      if (seconds_per_byte < kTuneMargin * t.base_score) {
+        t.best_grid = grid;
+        t.best_tile = tile;
+      }
+    } else if (++t.tries < kTuneTries) {
+      return;  // the cases changed between the two rounds: pair again
+    }
+    t.tries = 0;
+    const int n = t.phase == 0 ? kNumCandidates : kNumTileSizes - 1;
+    if (t.phase == 0 && t.next + 1 < n) {
+      t.next++;
+      return;
+    }
+    if (t.phase == 0) {
+      t.phase = 1;
+      t.next = 0;
+      return;
+    }
+    if (t.next + 1 < n) {
+      t.next++;
+      return;
+    }
     t.done = true;
     {
       std::lock_guard<std::mutex> g(g_tune_mu);
@@ -587,6 +629,7 @@ struct lbf_server {
       r->slot = -1;
     }
     dirty = true;
+    mix_version++;
     std::lock_guard<std::mutex> lk(mu);
     to_finish.push_back({r, rc});
     cv_help.notify_one();
@@ -692,10 +735,10 @@ struct lbf_server {
         }
       if (!n_res) continue;
       const uint32_t mask = type_mask();
-      bool measuring = false;
+      bool measuring = false, base = false;
       uint32_t grid = 0;
       int tile = kDefaultTileIdx;
-      choose_shape(mask, &measuring, &grid, &tile);
+      choose_shape(mask, &measuring, &base, &grid, &tile);
       if (tile != cur_tile) {
         cur_tile = tile;  // tile lists are per tile size
         dirty = true;
@@ -719,6 +762,7 @@ struct lbf_server {
           work += 2.0 * r->q * s * (double)r->nodes * (double)std::min<long long>(K, r->remaining);
           if (r->remaining <= K) last |= 1u << r->type;
         }
+      const uint64_t version = mix_version;  // the resident set this round runs
       hipGraphExec_t g = nullptr;
       if (!eager && (rc = get_graph(mask, last, K, grid, tile, &g))) {
         fail_all_resident(rc);
@@ -740,7 +784,7 @@ struct lbf_server {
       }
       float ms = 0.f;
       (void)hipEventElapsedTime(&ms, ev0, ev1);
-      if (measuring) record_tuning(mask, grid, tile, ms * 1e-3 / work);
+      if (measuring) record_tuning(mask, base, grid, tile, ms * 1e-3 / work, version, K);
       {
         std::lock_guard<std::mutex> lk(mu);
         rounds++;
