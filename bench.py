@@ -604,7 +604,9 @@ def main():
                 ("bf16s", abi.BF16_SHIFTED, abi.STREAM_AA, 2),
                 ("f32s", abi.F32_SHIFTED, abi.STREAM_AA, 4),
                 ("f32_two_array", abi.F32, abi.STREAM_TWO_ARRAY, 4),
-                ("f64_aa", abi.F64, abi.STREAM_AA, 8)]
+                ("f64_aa", abi.F64, abi.STREAM_AA, 8),
+                ("f32_two_step", abi.F32, abi.STREAM_TWO_STEP, 4),
+                ("f64_two_step", abi.F64, abi.STREAM_TWO_STEP, 8)]
     if args.only:
         by_name = {v[0]: v for v in variants}
         variants = [by_name[k] for k in args.only.split(",")]
@@ -627,7 +629,8 @@ def main():
                              wall_s=round(wall, 3), nonfinite=nonfinite, mass=mass,
                              bytes_per_site=2 * 19 * sbytes, storage_bytes=sbytes,
                              frac_of_sweep=round(gbs / sweep_gbs[sbytes], 4),
-                             streaming="aa" if streaming == abi.STREAM_AA else "two-array")
+                             streaming={abi.STREAM_AA: "aa", abi.STREAM_TWO_ARRAY: "two-array",
+                                        abi.STREAM_TWO_STEP: "two-step"}[streaming])
         print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)",
               file=sys.stderr, flush=True)
     # the same kernel on an all-fluid, fully periodic box: no bounce-back links,

@@ -108,6 +108,12 @@ extern "C" {
                                   face); else two arrays                         */
 #define LBF_STREAM_TWO_ARRAY 1 /* fused pull-stream + collide, ping-pong       */
 #define LBF_STREAM_AA 2        /* in-place AA pattern (one population array)   */
+#define LBF_STREAM_TWO_STEP 3  /* two arrays, two time steps per pass with the
+                                  first step's values held in LDS, where the
+                                  steps need no residuals; other steps as
+                                  LBF_STREAM_TWO_ARRAY, bit for bit the same
+                                  (3-D lattices, FP64 / FP32, faces no-slip,
+                                  bounce-back or periodic, no scalars)        */
 
 /* ---- source kinds (force / scalar source) ------------------------------- */
 #define LBF_SOURCE_NONE 0

@@ -43,7 +43,7 @@ PRECISION_BY_NAME = {v: k for k, v in PRECISION_NAMES.items()}
 BYTES_OF = {F64: 8, F32: 4, F32_SHIFTED: 4, F16_SHIFTED: 2, BF16_SHIFTED: 2}
 
 # streaming schemes
-STREAM_AUTO, STREAM_TWO_ARRAY, STREAM_AA = 0, 1, 2
+STREAM_AUTO, STREAM_TWO_ARRAY, STREAM_AA, STREAM_TWO_STEP = 0, 1, 2, 3
 
 SOURCE_NONE, SOURCE_UNIFORM, SOURCE_FIELD = 0, 1, 2
 

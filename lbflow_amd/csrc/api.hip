@@ -119,6 +119,22 @@ bool aa_reproduces_reference(const lbf_case_desc* d, const VsInfo& v) {
   return true;
 }
 
+// Two steps per pass (tb.hpp): 3-D lattices in FP64 / FP32 storage, faces
+// no-slip, moving bounce-back or periodic (opposite faces alike), no scalars
+// and no Shan-Chen force.
+bool tb_compatible(const lbf_case_desc* d, const VsInfo& v) {
+  if (v.d != 3 || d->n_scalars || d->multiphase) return false;
+  if (d->precision != LBF_F64 && d->precision != LBF_F32) return false;
+  for (int f = 0; f < 2 * v.d; f++) {
+    const int k = d->faces[f].kind;
+    if (k != LBF_BC_NO_SLIP && k != LBF_BC_BOUNCE_BACK && k != LBF_BC_PERIODIC) return false;
+  }
+  for (int a = 0; a < v.d; a++)
+    if ((d->faces[2 * a].kind == LBF_BC_PERIODIC) != (d->faces[2 * a + 1].kind == LBF_BC_PERIODIC))
+      return false;
+  return true;
+}
+
 bool get_launchers(int vs, int prec, Launchers* L) {
   switch (vs) {
     case LBF_D2Q9: return launchers_d2q9(prec, L);
@@ -214,6 +230,10 @@ struct lbf_case {
   ScalarDev sc[LBF_MAX_SCALARS];
   int ns = 0;
   bool aa = false;       // in-place AA scheme (one population array)
+  // two-array scheme with two steps per pass (tb.hpp) where the steps allow
+  // it: ts_grid workgroups of ts_zs z planes each
+  bool two_step = false;
+  int ts_grid = 0, ts_zs = 0;
   uint64_t host_step = 0;  // steps enqueued (device count, resynchronised on reads)
   // steps enqueued since the device step counter last advanced
   // (their launches carry it as step_offset; flush_steps advances the counter)
@@ -514,6 +534,16 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
                                     "no Shan-Chen force, faces in {no-slip, bounce-back, "
                                     "periodic, Zou-He})");
     }
+    if (d->streaming == LBF_STREAM_TWO_STEP) {
+      if (!tb_compatible(d, v)) {
+        delete c;
+        return fail(LBF_EUNSUPPORTED, "two-step passes need a 3-D lattice, FP64 or FP32 "
+                                      "storage, no scalars, no Shan-Chen force and faces in "
+                                      "{no-slip, bounce-back, periodic} with matching "
+                                      "periodicity of opposite faces");
+      }
+      c->two_step = true;
+    }
     if (d->streaming == LBF_STREAM_AA) c->aa = true;
     else if (d->streaming == LBF_STREAM_AUTO)
       c->aa = aa_ok && (c->prec != LBF_F64 || aa_reproduces_reference(d, v));
@@ -573,7 +603,26 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
   c->xcd_group = (c->aa && c->grid % 32 == 0) ? 4 : 0;
   {
     const char* env = getenv("LBF_STEP_GRAPHS");
-    c->graphs_on = N <= kGraphMaxNodes && !(env && env[0] == '0');
+    c->graphs_on = !c->two_step && N <= kGraphMaxNodes && !(env && env[0] == '0');
+  }
+  if (c->two_step) {
+    if ((unsigned long long)c->ld * q * c->sb >= (1ull << 32) || !c->L.tb_step) {
+      free_case(c);
+      return fail(LBF_EUNSUPPORTED, "two-step passes need population arrays below 4 GiB");
+    }
+    // tiles of tb_tx x tb_ty columns; the z axis is cut into segments so the
+    // grid holds about LBF_TB_WORKGROUPS (default 1024) workgroups: four
+    // rounds of one workgroup per CU (each holds most of a CU's LDS)
+    const long long tiles = (long long)((c->nx + c->L.tb_tx - 1) / c->L.tb_tx) *
+                            ((c->ny + c->L.tb_ty - 1) / c->L.tb_ty);
+    const char* ew = getenv("LBF_TB_WORKGROUPS");
+    const char* ez = getenv("LBF_TB_ZS");
+    const long long want = ew && atoll(ew) > 0 ? atoll(ew) : 1024;
+    long long zs = (c->nz * tiles + want - 1) / want;
+    if (ez && atoi(ez) > 0) zs = atoi(ez);
+    zs = std::max(1LL, std::min<long long>(zs, c->nz));
+    c->ts_zs = (int)zs;
+    c->ts_grid = (int)(tiles * ((c->nz + zs - 1) / zs));
   }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return cleanup(fail(LBF_EDEVICE, "hipStreamCreate failed"));
@@ -749,6 +798,7 @@ KArgs momentum_args(lbf_case* c) {
   a.fresh = c->aa_fresh ? 1 : 0;
   a.aa_odd = (int)(c->host_step & 1);
   a.xcd_group = c->aa ? c->xcd_group : 0;
+  a.tb_zs = c->ts_zs;
   a.inv_scale = 1.0 / c->scale;
   a.faces_noslip_only = 1;
   for (int f = 0; f < 2 * c->d; f++)
@@ -897,9 +947,36 @@ int step_graph(lbf_case* c, uint32_t flags, bool every_last, hipGraphExec_t* out
 // nsteps steps: captured batches where the case replays them (from an even,
 // non-fresh step, keeping the last step of a batch that stores moments out of
 // them), single launches otherwise.
+// One two-step pass (tb.hpp): steps without residuals or the stop rule, so
+// both steps defer their counter advance. The pass writes the other array,
+// and swapping the two pointers keeps the state in pop[step & 1].
+int enqueue_pass(lbf_case* c, bool last_of_batch) {
+  KArgs a = momentum_args(c);
+  a.mode = kModePull;
+  a.flags = last_of_batch ? kFlagStoreMoments : 0u;
+  a.res_mask = residual_mask(c);
+  a.step_offset = c->pending_steps;
+  HIPCHK(c->L.tb_step(a, c->ts_grid, c->stream));
+  std::swap(c->pop[0], c->pop[1]);
+  c->pending_steps += 2;
+  c->host_step += 2;
+  c->aa_fresh = false;
+  return 0;
+}
+
 int enqueue_steps(lbf_case* c, uint64_t nsteps, uint32_t flags, bool every_last) {
+  // passes need both steps' launches deferred and at most the second step's moments
+  const bool passes = c->two_step && !c->frozen && !c->needs_moments &&
+                      !(flags & (LBF_STEP_RESIDUALS | LBF_STEP_STOP_ON_CONVERGENCE)) &&
+                      !every_last;
   for (uint64_t k = 0; k < nsteps;) {
     const uint64_t left = nsteps - k;
+    if (passes && left >= 2) {
+      int rc = enqueue_pass(c, left == 2);
+      if (rc) return rc;
+      k += 2;
+      continue;
+    }
     if (c->graphs_on && !c->aa_fresh && (c->host_step & 1) == 0 &&
         (every_last ? left >= kGraphSteps : left > kGraphSteps)) {
       hipGraphExec_t g = nullptr;

@@ -69,6 +69,7 @@ struct KArgs {
   uint32_t step_offset;  // steps enqueued since the device step counter last advanced
   uint32_t plane_bytes;  // AA: bytes per population plane (array < 4 GiB)
   int xcd_group;         // runs of node blocks per XCD (0 = round-robin order)
+  int tb_zs;             // two-step passes: z planes per workgroup segment
   void* pop[2];          // ping-pong population buffers (two-array scheme)
   void* out;             // stream-only destination: f64 [q][ld] over every node, or
                          // with out_aos f64 [node - out_n0][q] over [out_n0, out_n1)

@@ -498,6 +498,27 @@ __device__ __forceinline__ void aa_faces(const KArgs& a, uint32_t on, T sc, T is
   });
 }
 
+// Moments and collision of one node in pair form (FP32 arithmetic on plain
+// or shifted storage): the in-place, two-array and two-step FP32 kernels all
+// call this, so they give the same bits. f: post-stream populations in the
+// compute units (shifted storage: scale * (f - w)); get_force(Fv) fills the
+// node's force and returns whether there is one (called once the velocity
+// sums are formed); moments(rho, u) receives rho and u shifted by F / 2 rho;
+// f* is returned in fs.
+template <class L, class St, int OP, class GetForce, class Moments>
+__device__ __forceinline__ void pair_collide(const KArgs& a, const typename St::T* f,
+                                             GetForce&& get_force, Moments&& moments,
+                                             typename St::T* fs) {
+#pragma clang fp contract(fast)
+  using T = typename St::T;
+  using k = K<L>;
+  constexpr int Q = L::Q, D = L::D;
+  const T omega = T(a.omega), omega_p = T(a.omega_p), wp = T(a.wp), wm = T(a.wm),
+          coeff_b = T(a.coeff_b);
+  const T sc = T(a.scale), isc = T(a.inv_scale);
+#include "pair_collide.inc"
+}
+
 // One node of the AA-pattern fused step (OP 0 BGK, 1 TRT, 2 MRT; ODD selects
 // the step parity) with its class word `cls` (kSolidBit past the last node).
 // `step` is the case's completed-step count, `fresh` marks an array that holds
@@ -561,164 +582,21 @@ __device__ __forceinline__ void aa_node_cls(const KArgs& a, uint32_t n, uint32_t
       store_moments(rho, u);
       ref_collide<L, OP>(a, f, rho, u, Fv, fs);
     } else {
-      // ---- moments over opposite pairs (i, ibar): sp = f_i + f_ibar, dm = f_i - f_ibar
-      // (this tier is not held to the reference's summation order)
-      T sp[Q], dm[Q];
-      T sum = f[0];
-      sfor<Q>([&](auto J) {
-        constexpr int i = decltype(J)::value;
-        if constexpr (pair_lead<L>(i)) {
-          sp[i] = f[i] + f[k::qbar(i)];
-          dm[i] = f[i] - f[k::qbar(i)];
-          sum += sp[i];
-        }
-      });
-      const T rho = St::shifted ? T(1) + sum * isc : sum;
-      const T irho = recip(rho);
-      const T ju = St::shifted ? isc * irho : irho;
-      T u[3];
-      sfor<D>([&](auto A) {
-        constexpr int ax = decltype(A)::value;
-        T s = T(0);
-        sfor<Q>([&](auto J) {
-          constexpr int i = decltype(J)::value;
-          if constexpr (pair_lead<L>(i)) {
-            constexpr int c = k::ca(i, ax);
-            if constexpr (c > 0) s += dm[i];
-            if constexpr (c < 0) s -= dm[i];
-          }
-        });
-        u[ax] = s * ju;
-      });
       // the uniform force and the per-node field are read in separate branches:
       // a select between their addresses would load the uniform value per lane
-      T Fv[3] = {T(0), T(0), T(0)};
-      if (a.force_kind == 1) {
-        sfor<D>([&](auto A) { Fv[decltype(A)::value] = T(a.F[decltype(A)::value]); });
-      } else if (a.force_kind) {
-        sfor<D>([&](auto A) {
-          constexpr int ax = decltype(A)::value;
-          Fv[ax] = T(global_ptr(a.src_field)[(size_t)ax * ld + n]);
-        });
-      }
-      if (a.force_kind)
-        sfor<D>([&](auto A) {
-          constexpr int ax = decltype(A)::value;
-          u[ax] += T(0.5) * Fv[ax] * irho;
-        });
-      store_moments(rho, u);
-      // ---- collision. f_eq,i = w_i (A + B t_i), t_i = 3 u.c_i + 4.5 (u.c_i)^2 - 1.5 u.u,
-      // with A = B = rho for plain storage and A = scale * drho, B = scale * rho in
-      // shifted units; over a pair, f_eq,i +- f_eq,ibar = 2 sym_i, 2 anti_i with
-      // sym_i = w_i (base + 4.5 B (u.c_i)^2), base = A - 1.5 B u.u, anti_i = 3 w_i B u.c_i
-      const T B = St::shifted ? sc * rho : rho;
-      const T base = (St::shifted ? sum : rho) - T(1.5) * B * udotu<L>(u);
-      if constexpr (OP == 1) {
-        // TRT (src/kernel.rs:44-65): f*_i = f_i - w+ (f+ - f_eq+) - w- (f- - f_eq-)
-        const T hp = T(0.5) * wp, hm = T(0.5) * wm;
-        fs[0] = f[0] - wp * (f[0] - T(k::w(0)) * base);
-        sfor<Q>([&](auto J) {
-          constexpr int i = decltype(J)::value;
-          if constexpr (pair_lead<L>(i)) {
-            constexpr int ib = k::qbar(i);
-            const T udc = udotc<L, i>(u);
-            const T sym2 = T(2.0 * k::w(i)) * base + (T(9.0 * k::w(i)) * B) * udc * udc;
-            const T anti2 = (T(6.0 * k::w(i)) * B) * udc;
-            const T S = hp * (sp[i] - sym2);
-            const T Dd = hm * (dm[i] - anti2);
-            fs[i] = f[i] - S - Dd;
-            fs[ib] = f[ib] - S + Dd;
-          }
-        });
-      } else if constexpr (OP == 0) {
-        // BGK (src/kernel.rs:28-42) with the Guo source (src/kernel.rs:122-154)
-        // S_i = b w_i (3 (c_i - u) + 9 (u.c_i) c_i).F, over a pair the parts
-        // b w_i (9 (u.c_i)(c_i.F) - 3 u.F) (even) and 3 b w_i c_i.F (odd)
-        const bool force = a.force_kind != 0;
-        const T bs = St::shifted ? coeff_b * sc : coeff_b;
-        const T uF = udotu_F<L>(u, Fv);
-        fs[0] = omega_p * f[0] + omega * (T(k::w(0)) * base);
-        if (force) fs[0] += bs * T(k::w(0)) * (T(-3.0) * uF);
-        sfor<Q>([&](auto J) {
-          constexpr int i = decltype(J)::value;
-          if constexpr (pair_lead<L>(i)) {
-            constexpr int ib = k::qbar(i);
-            const T udc = udotc<L, i>(u);
-            const T sym = T(k::w(i)) * base + (T(4.5 * k::w(i)) * B) * udc * udc;
-            const T anti = (T(3.0 * k::w(i)) * B) * udc;
-            T gi = omega_p * f[i] + omega * (sym + anti);
-            T gb = omega_p * f[ib] + omega * (sym - anti);
-            if (force) {
-              const T cF = udotc<L, i>(Fv);
-              const T se = (bs * T(k::w(i))) * (T(9.0) * udc * cF - T(3.0) * uF);
-              const T so = (bs * T(3.0 * k::w(i))) * cF;
-              gi += se + so;
-              gb += se - so;
-            }
-            fs[i] = gi;
-            fs[ib] = gb;
-          }
-        });
-      } else {
-        // MRT (src/kernel.rs:67-120): f* = M^-1 (m - s (m - m_eq)), every
-        // product over pair sums / differences (each row of M has a parity)
-        static_assert(mrt_rows_have_parity<L>(), "pair-split MRT needs rows of fixed parity");
-        T s[Q], ms[Q];
-        sfor<Q>([&](auto KI) { s[decltype(KI)::value] = T(a.s[decltype(KI)::value]); });
-        if constexpr (L::id == 1 || L::id == 2 || L::id == 3) {
-          // m_eq from the reference's closed forms (shifted: scaled, minus M w)
-          T m[Q], meq[Q];
-          sfor<Q>([&](auto Kr) {
-            m[decltype(Kr)::value] = mrt_dot_pairs<L, decltype(Kr)::value>(f[0], sp, dm);
+      auto get_force = [&](T* Fv) {
+        if (a.force_kind == 1) {
+          sfor<D>([&](auto A) { Fv[decltype(A)::value] = T(a.F[decltype(A)::value]); });
+        } else if (a.force_kind) {
+          sfor<D>([&](auto A) {
+            constexpr int ax = decltype(A)::value;
+            Fv[ax] = T(global_ptr(a.src_field)[(size_t)ax * ld + n]);
           });
-          if constexpr (St::shifted)
-            meq_shifted<L>(sum, B, sc, u, nullptr, meq);
-          else
-            meq_closed<L, T>(rho, u, nullptr, meq);
-          sfor<Q>([&](auto Kr) {
-            constexpr int kk = decltype(Kr)::value;
-            ms[kk] = m[kk] - s[kk] * (m[kk] - meq[kk]);
-          });
-          mrt_inv_pairs<L>(ms, fs);
-        } else {
-          // no closed form, m_eq = M f_eq: m - s (m - m_eq) = M f - s M (f - f_eq),
-          // so f* = f - M^-1 (s M (f - f_eq)), two products on f - f_eq
-          T dfs[Q];
-          if constexpr (L::id == 4) {
-            // tensor-product basis: separable transforms on f - f_eq
-            T neq[Q];
-            neq[0] = f[0] - T(k::w(0)) * base;
-            sfor<Q>([&](auto J) {
-              constexpr int i = decltype(J)::value;
-              if constexpr (pair_lead<L>(i)) {
-                const T udc = udotc<L, i>(u);
-                const T sym = T(k::w(i)) * base + (T(4.5 * k::w(i)) * B) * udc * udc;
-                const T anti = (T(3.0 * k::w(i)) * B) * udc;
-                neq[i] = f[i] - (sym + anti);
-                neq[k::qbar(i)] = f[k::qbar(i)] - (sym - anti);
-              }
-            });
-            mrt_tensor_relax<L>(neq, s, dfs);
-          } else {
-            T nsp[Q], ndm[Q];
-            const T n0 = f[0] - T(k::w(0)) * base;
-            sfor<Q>([&](auto J) {
-              constexpr int i = decltype(J)::value;
-              if constexpr (pair_lead<L>(i)) {
-                const T udc = udotc<L, i>(u);
-                nsp[i] = sp[i] - (T(2.0 * k::w(i)) * base + (T(9.0 * k::w(i)) * B) * udc * udc);
-                ndm[i] = dm[i] - (T(6.0 * k::w(i)) * B) * udc;
-              }
-            });
-            sfor<Q>([&](auto Kr) {
-              constexpr int kk = decltype(Kr)::value;
-              ms[kk] = s[kk] * mrt_dot_pairs<L, kk>(n0, nsp, ndm);
-            });
-            mrt_inv_pairs<L>(ms, dfs);
-          }
-          sfor<Q>([&](auto J) { fs[decltype(J)::value] = f[decltype(J)::value] - dfs[decltype(J)::value]; });
         }
-      }
+        return a.force_kind != 0;
+      };
+      auto& moments = store_moments;
+#include "pair_collide.inc"
     }
     // ---- scatter
     const uint32_t own = n * (uint32_t)sizeof(S);

@@ -23,6 +23,8 @@
 // launches can be replayed.
 #pragma once
 
+#include <type_traits>
+
 #include "engine.hpp"
 #include "node_math.hpp"
 
@@ -294,6 +296,12 @@ __device__ __forceinline__ void readback_range(const KArgs& a, long long* n0, lo
   *n1 = r ? a.out_n1 : a.N;
 }
 
+// pair-form moments and collision of the FP32 kernels (fast.hpp)
+template <class L, class St, int OP, class GetForce, class Moments>
+__device__ __forceinline__ void pair_collide(const KArgs& a, const typename St::T* f,
+                                             GetForce&& get_force, Moments&& moments,
+                                             typename St::T* fs);
+
 // Plain-form momentum step (storage holds f), two-array scheme: FP64 and
 // FP32 cases the in-place scheme does not take (Shan-Chen, anti-bounce-back,
 // Zou-He edges in FP64), or on request.
@@ -356,8 +364,12 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
       readback_store<Q>(a, n, f);
       continue;
     }
+    // FP32 storage collides in pair form (pair_collide, shared with the
+    // in-place and two-step kernels, so the three schemes give the same bits);
+    // FP64 in the reference's order
+    constexpr bool pair = !std::is_same<St, StF64>::value;
     T rho, u[3];
-    moments<L>(f, rho, u);
+    if (!pair || a.mode == kModePhi) moments<L>(f, rho, u);
     if (a.mode == kModePhi) {
       // compute_phi (src/momentum/multiphase.rs:48-54): psi = 1 - exp(-rho)
       phi[n] = T(1) * (T(1) - exp(-rho / T(1)));
@@ -396,11 +408,25 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
       } else {
         sfor<D>([&](auto A) { F[decltype(A)::value] = B[decltype(A)::value]; });
       }
-      sfor<D>([&](auto A) {
-        constexpr int ax = decltype(A)::value;
-        u[ax] += T(0.5) * F[ax] / rho;  // node.rs:450-455
-      });
+      if constexpr (!pair)
+        sfor<D>([&](auto A) {
+          constexpr int ax = decltype(A)::value;
+          u[ax] += T(0.5) * F[ax] / rho;  // node.rs:450-455
+        });
     }
+    T fs[Q];
+    if constexpr (pair)
+      pair_collide<L, St, OP>(
+          a, f,
+          [&](T* Fv) {
+            sfor<D>([&](auto A) { Fv[decltype(A)::value] = F[decltype(A)::value]; });
+            return has_force;
+          },
+          [&](T r, const T* uu) {
+            rho = r;
+            sfor<D>([&](auto A) { u[decltype(A)::value] = uu[decltype(A)::value]; });
+          },
+          fs);
     if (store_m) {
       rho_c[n] = rho;
       sfor<D>([&](auto A) {
@@ -420,20 +446,22 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
         acc[4] += bad ? 1.0 : 0.0;
       }
     }
-    T feq[Q], fs[Q];
-    equilibrium<L>(rho, u, feq);
-    if constexpr (OP == 0) {
-      bgk<L>(f, feq, omega, omega_p, fs);
-      if (has_force) add_guo<L>(u, F, coeff_b, fs);
-    } else if constexpr (OP == 1) {
-      trt<L>(f, feq, wp, wm, fs);
-    } else {
-      T s[Q];
-      sfor<Q>([&](auto KI) {
-        constexpr int kk = decltype(KI)::value;
-        s[kk] = T(a.s[kk]);
-      });
-      mrt<L>(rho, u, f, feq, s, fs);
+    if constexpr (!pair) {
+      T feq[Q];
+      equilibrium<L>(rho, u, feq);
+      if constexpr (OP == 0) {
+        bgk<L>(f, feq, omega, omega_p, fs);
+        if (has_force) add_guo<L>(u, F, coeff_b, fs);
+      } else if constexpr (OP == 1) {
+        trt<L>(f, feq, wp, wm, fs);
+      } else {
+        T s[Q];
+        sfor<Q>([&](auto KI) {
+          constexpr int kk = decltype(KI)::value;
+          s[kk] = T(a.s[kk]);
+        });
+        mrt<L>(rho, u, f, feq, s, fs);
+      }
     }
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;

@@ -5,6 +5,7 @@
 #include "fast.hpp"
 #include "kernels.hpp"
 #include "serve.hpp"
+#include "tb.hpp"
 
 namespace lbf {
 
@@ -64,6 +65,33 @@ static hipError_t aa_init_t(const KArgs& a, int grid, hipStream_t s) {
   return launch(momentum_init_aa<L, St>, a, grid, s);
 }
 
+template <class L, class St>
+static hipError_t tb_step_t(const KArgs& a, int grid, hipStream_t s) {
+  constexpr int nt = TbShape<L, St>::NT;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(nt), 0, s, a);
+    return hipGetLastError();
+  };
+  if (a.flags & kFlagStoreMoments) {
+    if (a.op == 0) return go(momentum_step_tb2<L, St, 0, true>);
+    if (a.op == 1) return go(momentum_step_tb2<L, St, 1, true>);
+    if constexpr (L::has_mrt) return go(momentum_step_tb2<L, St, 2, true>);
+  } else {
+    if (a.op == 0) return go(momentum_step_tb2<L, St, 0, false>);
+    if (a.op == 1) return go(momentum_step_tb2<L, St, 1, false>);
+    if constexpr (L::has_mrt) return go(momentum_step_tb2<L, St, 2, false>);
+  }
+  return hipErrorInvalidValue;
+}
+template <class L, class St>
+static void tb_launchers(Launchers* out) {
+  if constexpr (L::D == 3) {
+    using Sh = TbShape<L, St>;
+    out->tb_step = tb_step_t<L, St>;
+    out->tb_tx = Sh::TX, out->tb_ty = Sh::TY, out->tb_threads = Sh::NT;
+  }
+}
+
 // serving-tier batched steps exist for the lattices of the serving mix
 template <class L, class St>
 static void serve_launchers(Launchers* out) {
@@ -95,6 +123,7 @@ static bool make_launchers(int precision, Launchers* out) {
       out->aa_step = aa_step_t<L, StF64>;
       out->aa_init = aa_init_t<L, StF64>;
       out->aa_readback = aa_readback_t<L, StF64>;
+      tb_launchers<L, StF64>(out);
       return true;
     case 1:  // LBF_F32
       out->mom_step = mom_step_plain<L, StF32>;
@@ -104,6 +133,7 @@ static bool make_launchers(int precision, Launchers* out) {
       out->aa_step = aa_step_t<L, StF32>;
       out->aa_init = aa_init_t<L, StF32>;
       out->aa_readback = aa_readback_t<L, StF32>;
+      tb_launchers<L, StF32>(out);
       serve_launchers<L, StF32>(out);
       return true;
     case 2:  // LBF_F32_SHIFTED

@@ -76,6 +76,7 @@ pub const LBF_BF16_SHIFTED: i32 = 4;
 pub const LBF_STREAM_AUTO: i32 = 0;
 pub const LBF_STREAM_TWO_ARRAY: i32 = 1;
 pub const LBF_STREAM_AA: i32 = 2;
+pub const LBF_STREAM_TWO_STEP: i32 = 3;
 
 // sources (force / scalar source)
 pub const LBF_SOURCE_NONE: i32 = 0;
