@@ -233,7 +233,7 @@ struct lbf_case {
   // two-array scheme with two steps per pass (tb.hpp) where the steps allow
   // it: ts_grid workgroups of ts_zs z planes each
   bool two_step = false;
-  int ts_grid = 0, ts_zs = 0;
+  int ts_grid = 0, ts_zs = 0, ts_shape = 0;
   uint64_t host_step = 0;  // steps enqueued (device count, resynchronised on reads)
   // steps enqueued since the device step counter last advanced
   // (their launches carry it as step_offset; flush_steps advances the counter)
@@ -610,14 +610,23 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
       free_case(c);
       return fail(LBF_EUNSUPPORTED, "two-step passes need population arrays below 4 GiB");
     }
-    // tiles of tb_tx x tb_ty columns; the z axis is cut into segments so the
-    // grid holds about LBF_TB_WORKGROUPS (default 1024) workgroups: four
-    // rounds of one workgroup per CU (each holds most of a CU's LDS)
-    const long long tiles = (long long)((c->nx + c->L.tb_tx - 1) / c->L.tb_tx) *
-                            ((c->ny + c->L.tb_ty - 1) / c->L.tb_ty);
+    // tiles of tb_tx x tb_ty columns (shape 1: rows of 64 nodes where the
+    // ring fits; LBF_TB_SHAPE=0 takes 32 x 16 tiles); the z axis is cut into
+    // segments so the grid holds about one workgroup per CU (each holds most
+    // of a CU's LDS): 256^3 D3Q19 FP32 measured 428 us per step with 256
+    // workgroups, 450-460 us with 512 or 1024 (more ring planes re-read and
+    // a second round); LBF_TB_WORKGROUPS sets another count
+    const char* es = getenv("LBF_TB_SHAPE");
+    c->ts_shape = es && es[0] == '0' ? 0 : 1;
+    const int tx = c->L.tb_tx[c->ts_shape], ty = c->L.tb_ty[c->ts_shape];
+    const long long tiles = (long long)((c->nx + tx - 1) / tx) * ((c->ny + ty - 1) / ty);
     const char* ew = getenv("LBF_TB_WORKGROUPS");
     const char* ez = getenv("LBF_TB_ZS");
-    const long long want = ew && atoll(ew) > 0 ? atoll(ew) : 1024;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    const long long want = ew && atoll(ew) > 0 ? atoll(ew) : cus;
     long long zs = (c->nz * tiles + want - 1) / want;
     if (ez && atoi(ez) > 0) zs = atoi(ez);
     zs = std::max(1LL, std::min<long long>(zs, c->nz));
@@ -799,6 +808,7 @@ KArgs momentum_args(lbf_case* c) {
   a.aa_odd = (int)(c->host_step & 1);
   a.xcd_group = c->aa ? c->xcd_group : 0;
   a.tb_zs = c->ts_zs;
+  a.tb_shape = c->ts_shape;
   a.inv_scale = 1.0 / c->scale;
   a.faces_noslip_only = 1;
   for (int f = 0; f < 2 * c->d; f++)

@@ -25,10 +25,11 @@ struct Launchers {
   LaunchFn aa_step = nullptr;   // in-place AA-pattern step (momentum only)
   LaunchFn aa_init = nullptr;
   LaunchFn aa_readback = nullptr;
-  // two steps per pass over the two-array scheme (3-D lattices, FP64 / FP32);
-  // launched with tb_threads threads per workgroup, tiles of tb_tx x tb_ty
+  // two steps per pass over the two-array scheme (3-D lattices, FP64 / FP32),
+  // tiles of tb_tx x tb_ty columns in tile shape KArgs::tb_shape (1: rows of
+  // 64 nodes where they fit)
   LaunchFn tb_step = nullptr;
-  int tb_tx = 0, tb_ty = 0, tb_threads = 0;
+  int tb_tx[2] = {0, 0}, tb_ty[2] = {0, 0};
   // [BGK, TRT][tile size 128, 256, 512] (D2Q9 / D3Q19, AA storages)
   ServeLaunchFn serve[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};
   ServeLastFn serve_last[2][3] = {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}};

@@ -70,6 +70,7 @@ struct KArgs {
   uint32_t plane_bytes;  // AA: bytes per population plane (array < 4 GiB)
   int xcd_group;         // runs of node blocks per XCD (0 = round-robin order)
   int tb_zs;             // two-step passes: z planes per workgroup segment
+  int tb_shape;          // two-step passes: tile shape (1: rows of 64 nodes where they fit)
   void* pop[2];          // ping-pong population buffers (two-array scheme)
   void* out;             // stream-only destination: f64 [q][ld] over every node, or
                          // with out_aos f64 [node - out_n0][q] over [out_n0, out_n1)

@@ -65,30 +65,34 @@ static hipError_t aa_init_t(const KArgs& a, int grid, hipStream_t s) {
   return launch(momentum_init_aa<L, St>, a, grid, s);
 }
 
-template <class L, class St>
-static hipError_t tb_step_t(const KArgs& a, int grid, hipStream_t s) {
-  constexpr int nt = TbShape<L, St>::NT;
+template <class L, class St, int SHAPE>
+static hipError_t tb_step_s(const KArgs& a, int grid, hipStream_t s) {
+  constexpr int nt = TbShape<L, St, SHAPE>::NT;
   auto go = [&](auto kernel) {
     hipLaunchKernelGGL(kernel, dim3(grid), dim3(nt), 0, s, a);
     return hipGetLastError();
   };
   if (a.flags & kFlagStoreMoments) {
-    if (a.op == 0) return go(momentum_step_tb2<L, St, 0, true>);
-    if (a.op == 1) return go(momentum_step_tb2<L, St, 1, true>);
-    if constexpr (L::has_mrt) return go(momentum_step_tb2<L, St, 2, true>);
+    if (a.op == 0) return go(momentum_step_tb2<L, St, 0, true, SHAPE>);
+    if (a.op == 1) return go(momentum_step_tb2<L, St, 1, true, SHAPE>);
+    if constexpr (L::has_mrt) return go(momentum_step_tb2<L, St, 2, true, SHAPE>);
   } else {
-    if (a.op == 0) return go(momentum_step_tb2<L, St, 0, false>);
-    if (a.op == 1) return go(momentum_step_tb2<L, St, 1, false>);
-    if constexpr (L::has_mrt) return go(momentum_step_tb2<L, St, 2, false>);
+    if (a.op == 0) return go(momentum_step_tb2<L, St, 0, false, SHAPE>);
+    if (a.op == 1) return go(momentum_step_tb2<L, St, 1, false, SHAPE>);
+    if constexpr (L::has_mrt) return go(momentum_step_tb2<L, St, 2, false, SHAPE>);
   }
   return hipErrorInvalidValue;
 }
 template <class L, class St>
+static hipError_t tb_step_t(const KArgs& a, int grid, hipStream_t s) {
+  return a.tb_shape ? tb_step_s<L, St, 1>(a, grid, s) : tb_step_s<L, St, 0>(a, grid, s);
+}
+template <class L, class St>
 static void tb_launchers(Launchers* out) {
   if constexpr (L::D == 3) {
-    using Sh = TbShape<L, St>;
     out->tb_step = tb_step_t<L, St>;
-    out->tb_tx = Sh::TX, out->tb_ty = Sh::TY, out->tb_threads = Sh::NT;
+    out->tb_tx[0] = TbShape<L, St, 0>::TX, out->tb_ty[0] = TbShape<L, St, 0>::TY;
+    out->tb_tx[1] = TbShape<L, St, 1>::TX, out->tb_ty[1] = TbShape<L, St, 1>::TY;
   }
 }
 

@@ -28,15 +28,17 @@ namespace lbf {
 
 // Tile shape by bytes per node (Q * storage size): the LDS ring of three
 // planes of (TX + 2) x (TY + 2) nodes, plus their class words, fits one CU's
-// 160 KiB.
-template <class L, class St>
+// 160 KiB. SHAPE 1 takes rows twice as wide (64 nodes) where that fits.
+template <class L, class St, int SHAPE>
 struct TbShape {
   static constexpr int BPN = L::Q * (int)sizeof(typename St::S);
-  static constexpr int TX = BPN > 152 ? 16 : 32;
-  static constexpr int TY = BPN <= 80 ? 16 : 8;
+  static constexpr int TX = BPN > 152 ? 16 : (SHAPE == 1 && BPN <= 80 ? 64 : 32);
+  static constexpr int TY = BPN <= 80 ? (SHAPE == 1 ? 8 : 16) : 8;
   static constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
   static constexpr int NT = (HN + 63) / 64 * 64;  // one thread per ring-plane node
   static constexpr int CORE = TX * TY;
+  // waves per SIMD of the one workgroup on a CU (registers are budgeted for them)
+  static constexpr int WAVES_PER_SIMD = (NT / 64 + 3) / 4;
   static constexpr long LDS = 3L * HN * (BPN + 4);
   static_assert(NT <= 1024 && CORE <= NT, "tile shape");
   static_assert(LDS <= 160L * 1024, "LDS ring exceeds a CU");
@@ -134,13 +136,14 @@ __device__ __forceinline__ uint32_t tb_tile(uint32_t b, uint32_t nb) {
 // step t+1 into pop[(t & 1) ^ 1]; the host swaps the two pointers after
 // enqueueing the pass, so the state is again in pop[(t + 2) & 1]. MOM stores
 // the second step's moments into rho / u[(t + 1) & 1].
-template <class L, class St, int OP, bool MOM>
-__global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St>::NT)))
+template <class L, class St, int OP, bool MOM, int SHAPE>
+__global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHAPE>::NT),
+                               amdgpu_waves_per_eu(TbShape<L, St, SHAPE>::WAVES_PER_SIMD)))
     momentum_step_tb2(KArgs a) {
   using T = typename St::T;
   using S = typename St::S;
   using k = K<L>;
-  using Sh = TbShape<L, St>;
+  using Sh = TbShape<L, St, SHAPE>;
   constexpr int Q = L::Q, TX = Sh::TX, TY = Sh::TY, HX = Sh::HX, HN = Sh::HN;
   static_assert(L::D == 3, "two-step passes march along z");
   __shared__ S ring[3][Q][HN];
@@ -242,16 +245,13 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St>::NT
       v[j] = bload<S>(rs, opaque((sel >> jb & 1u) ? bb : nb), 0u);
     });
   };
-  // first step of the loaded plane into ring slot `slot`
-  auto first = [&](int slot) {
-    if (t < HN) cring[slot][t] = v_cls;
-    if (v_cls & kSolidBit) return;
-    T f[Q], fs[Q], rho, u[3];
-    sfor<Q>([&](auto J) { f[decltype(J)::value] = T(v[decltype(J)::value]); });
-    // the gather loaded f*_i into v[ibar] on every face-crossing link i
-    if (!fresh && v_on)
-      tb_faces<L, T>(a, v_on, f, [&](auto I) { return T(v[k::qbar(decltype(I)::value)]); });
-    tb_collide<L, St, OP>(a, v_n, f, fs, rho, u);
+  // first step of a taken plane (post-stream f, class word, node) into ring
+  // slot `slot`
+  auto first = [&](int slot, const T* f, uint32_t cls, uint32_t n) {
+    if (t < HN) cring[slot][t] = cls;
+    if (cls & kSolidBit) return;
+    T fs[Q], rho, u[3];
+    tb_collide<L, St, OP>(a, n, f, fs, rho, u);
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;
       ring[slot][j][t] = static_cast<S>(fs[j]);
@@ -304,19 +304,47 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St>::NT
   };
 
   // ---- march: iteration i runs the first step on plane z0 - 1 + i and the
-  // second step on plane z0 - 2 + i (from i = 2)
+  // second step on plane z0 - 2 + i (from i = 2). The next plane's loads go
+  // out after the first barrier (kEarly: before the first step's arithmetic,
+  // which measured 5 % slower in FP32 and spills in FP64).
+  constexpr bool kEarly = false;
   const int np = (z1 - z0) + 2;
   uint32_t c_next = class_of(z0);
   issue(z0 - 1, class_of(z0 - 1));
   for (int i = 0; i < np; i++) {
     const int slot = i % 3;
-    first(slot);
-    __syncthreads();  // ring slot `slot` complete
-    if (i + 1 < np) {
-      const uint32_t c = c_next;
-      if (i + 2 < np) c_next = class_of(z0 + i + 1);
-      issue(z0 + i, c);
+    // take this plane's loaded values (post-stream f after the faces)
+    auto take = [&](T* f) {
+      if (v_cls & kSolidBit) return;
+      sfor<Q>([&](auto J) { f[decltype(J)::value] = T(v[decltype(J)::value]); });
+      // the gather loaded f*_i into v[ibar] on every face-crossing link i
+      if (!fresh && v_on)
+        tb_faces<L, T>(a, v_on, f, [&](auto I) { return T(v[k::qbar(decltype(I)::value)]); });
+    };
+    auto next = [&]() {
+      if (i + 1 < np) {
+        const uint32_t c = c_next;
+        if (i + 2 < np) c_next = class_of(z0 + i + 1);
+        issue(z0 + i, c);
+      }
+    };
+    if constexpr (kEarly) {
+      // the next plane's loads are in flight across this plane's first step,
+      // the barriers and the second step
+      T f[Q];
+      const uint32_t cls = v_cls, n1 = v_n;
+      take(f);
+      next();
+      first(slot, f, cls, n1);
+    } else {
+      [&]() {
+        T f[Q];
+        take(f);
+        first(slot, f, v_cls, v_n);
+      }();
     }
+    __syncthreads();  // ring slot `slot` complete
+    if constexpr (!kEarly) next();
     if (i >= 2) second(z0 - 2 + i, (i + 1) % 3, (i + 2) % 3, slot);
     __syncthreads();  // second step's reads done before the next first step overwrites
   }

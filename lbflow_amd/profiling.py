@@ -30,7 +30,7 @@ _STORAGE = {"StF32": "f32", "StF16S": "f16s", "StBF16S": "bf16s", "StF32S": "f32
 
 def variant_of(kernel_name):
     """bench.py variant a step kernel belongs to, or None."""
-    m = re.search(r"momentum_step_(aa|plain)<lbf::(\w+), lbf::(\w+),", kernel_name)
+    m = re.search(r"momentum_step_(aa|plain|tb2)<lbf::(\w+), lbf::(\w+),", kernel_name)
     if not m:
         return None
     scheme, _lat, storage = m.groups()
@@ -39,7 +39,11 @@ def variant_of(kernel_name):
         return None
     if scheme == "plain":
         return "f64" if v == "f64" else v + "_two_array"
-    return "f64_aa" if v == "f64" else v
+    if scheme == "tb2":
+        return "f64_two_step" if v == "f64" else v
+    if v == "f64":
+        return "f64_aa"
+    return "f32_aa" if v == "f32" else v
 
 
 def short_name(kernel_name):
@@ -99,18 +103,22 @@ def traffic_summary(fetch_csv, write_csv, copy_bytes, cmd_fetch, cmd_write, node
     by_variant = collections.defaultdict(list)
     for k, (rb, wb, cnt) in per.items():
         v = variant_of(k)
-        s = {"f32": 4, "f16s": 2, "bf16s": 2, "f32s": 4, "f64": 8, "f64_aa": 8,
-             "f32_two_array": 4}.get(v)
-        alg = 2.0 * q * nodes * s if s else None
+        s = {"f32": 4, "f32_aa": 4, "f16s": 2, "bf16s": 2, "f32s": 4, "f64": 8, "f64_aa": 8,
+             "f32_two_array": 4, "f64_two_step": 8}.get(v)
+        # a two-step launch (momentum_step_tb2) advances two time steps
+        steps = 2 if "momentum_step_tb2" in k else 1
+        alg = 2.0 * q * nodes * s * steps if s else None
         ratio = f"{(rb + wb) / alg:6.2f}" if alg else "     -"
         out.append(f"{short_name(k):88s} {cnt:3d} {rb / 1e9:8.3f} {wb / 1e9:8.3f} "
                    f"{(alg or 0) / 1e9:7.3f} {ratio}")
         # streaming steps (no moment store) are the launches a run is made of
         mom = re.search(r"momentum_step_aa<[^>]*, (true|false), (true|false)>", k)
-        if v and not (mom and mom.group(2) == "true"):
+        mom2 = re.search(r"momentum_step_tb2<[^>]*, (true|false), \d+>", k)
+        if v and not (mom and mom.group(2) == "true") and not (mom2 and mom2.group(1) == "true"):
             by_variant[v].append(rb + wb)
     out.append("")
-    out.append("HBM bytes per step launch (mean of the even and odd streaming kernels):")
+    out.append("HBM bytes per step launch (mean of the streaming kernels: in place, the even "
+               "and odd steps; two-step, one launch per two steps):")
     for v, xs in sorted(by_variant.items()):
         out.append(f"traffic_bytes_per_launch {v} {sum(xs) / len(xs):.0f}")
     return "\n".join(out) + "\n"
