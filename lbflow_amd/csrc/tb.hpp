@@ -26,22 +26,59 @@
 
 namespace lbf {
 
-// Tile shape by bytes per node (Q * storage size): the LDS ring of three
-// planes of (TX + 2) x (TY + 2) nodes, plus their class words, fits one CU's
-// 160 KiB. SHAPE 1 takes rows twice as wide (64 nodes) where that fits.
+// The LDS ring holds the first step's post-collision values of the tile and
+// its one-node ring, HN = (TX + 2) x (TY + 2) nodes per plane. The second
+// step on plane p reads the directions with c_z = +1 from plane p - 1, c_z = 0
+// and every bounce-back value from plane p, and c_z = -1 from plane p + 1. So
+// the c_z = +1 directions live in a ring of four planes and the others in a
+// ring of three: the first step of the next plane then writes nothing the
+// second step of this one still reads, and one barrier per plane suffices.
+template <class L>
+constexpr int tb_up_dirs() {
+  int n = 0;
+  for (int i = 0; i < L::Q; i++) n += L::c[i][2] > 0 ? 1 : 0;
+  return n;
+}
+// position of direction j in its ring (c_z = +1 ring or the other)
+template <class L>
+constexpr int tb_ring_index(int j) {
+  int n = 0;
+  for (int i = 0; i < j; i++) n += (L::c[i][2] > 0) == (L::c[j][2] > 0) ? 1 : 0;
+  return n;
+}
+template <class L, class St>
+constexpr long tb_lds_bytes(int tx, int ty) {
+  return (long)(4 * tb_up_dirs<L>() + 3 * (L::Q - tb_up_dirs<L>())) * (tx + 2) * (ty + 2) *
+         (long)sizeof(typename St::S);
+}
+constexpr long kTbLds = 160L * 1024;
+// Tile shape: the first of these (TX, TY) whose ring fits one CU's 160 KiB.
+// Wide rows first (the loads of a wavefront then touch fewer cache lines;
+// 256^3 D3Q19 FP32: 64 x 8 417 us per step, 32 x 16 432 us), then square
+// tiles of 128-byte rows for 8-byte storage.
+constexpr int kTbShapes[2][8][2] = {
+    {{32, 16}, {32, 8}, {16, 16}, {16, 8}, {32, 4}, {16, 4}, {16, 2}, {8, 2}},
+    {{64, 8}, {32, 8}, {16, 16}, {16, 8}, {32, 4}, {16, 4}, {16, 2}, {8, 2}}};
+template <class L, class St, int SHAPE>
+constexpr int tb_shape_index() {
+  for (int k = 0; k < 8; k++)
+    if (tb_lds_bytes<L, St>(kTbShapes[SHAPE][k][0], kTbShapes[SHAPE][k][1]) <= kTbLds) return k;
+  return 7;
+}
 template <class L, class St, int SHAPE>
 struct TbShape {
-  static constexpr int BPN = L::Q * (int)sizeof(typename St::S);
-  static constexpr int TX = BPN > 152 ? 16 : (SHAPE == 1 && BPN <= 80 ? 64 : 32);
-  static constexpr int TY = BPN <= 80 ? (SHAPE == 1 ? 8 : 16) : 8;
+  static constexpr int TX = kTbShapes[SHAPE][tb_shape_index<L, St, SHAPE>()][0];
+  static constexpr int TY = kTbShapes[SHAPE][tb_shape_index<L, St, SHAPE>()][1];
   static constexpr int HX = TX + 2, HY = TY + 2, HN = HX * HY;
   static constexpr int NT = (HN + 63) / 64 * 64;  // one thread per ring-plane node
   static constexpr int CORE = TX * TY;
+  static constexpr int NUP = tb_up_dirs<L>();    // directions with c_z = +1
+  static constexpr int NREST = L::Q - NUP;
   // waves per SIMD of the one workgroup on a CU (registers are budgeted for them)
   static constexpr int WAVES_PER_SIMD = (NT / 64 + 3) / 4;
-  static constexpr long LDS = 3L * HN * (BPN + 4);
+  static constexpr long LDS = tb_lds_bytes<L, St>(TX, TY);
   static_assert(NT <= 1024 && CORE <= NT, "tile shape");
-  static_assert(LDS <= 160L * 1024, "LDS ring exceeds a CU");
+  static_assert(LDS <= kTbLds, "LDS ring exceeds a CU");
 };
 
 // face conditions of kinds no-slip (1) and moving bounce-back (2) in face
@@ -146,8 +183,19 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
   using Sh = TbShape<L, St, SHAPE>;
   constexpr int Q = L::Q, TX = Sh::TX, TY = Sh::TY, HX = Sh::HX, HN = Sh::HN;
   static_assert(L::D == 3, "two-step passes march along z");
-  __shared__ S ring[3][Q][HN];
-  __shared__ uint32_t cring[3][HN];
+  constexpr int NUP = Sh::NUP, NREST = Sh::NREST;
+  // ring of the c_z = +1 directions (4 planes), then the others (3 planes)
+  __shared__ S ring[4 * NUP * HN + 3 * NREST * HN];
+  // element index of direction J (std::integral_constant) of ring node h in
+  // the slot of march iteration `it` (the iteration that wrote that plane)
+  auto at = [&](auto J, int it, int h) -> int {
+    constexpr int j = decltype(J)::value;
+    constexpr int idx = tb_ring_index<L>(j);
+    if constexpr (L::c[j][2] > 0)
+      return ((it & 3) * NUP + idx) * HN + h;
+    else
+      return 4 * NUP * HN + ((it % 3) * NREST + idx) * HN + h;
+  };
 
   const Ctrl* ctrl = a.ctrl;
   if (ctrl->stopped) return;
@@ -247,46 +295,46 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
   };
   // first step of a taken plane (post-stream f, class word, node) into ring
   // slot `slot`
-  auto first = [&](int slot, const T* f, uint32_t cls, uint32_t n) {
-    if (t < HN) cring[slot][t] = cls;
+  auto first = [&](int it, const T* f, uint32_t cls, uint32_t n) {
     if (cls & kSolidBit) return;
     T fs[Q], rho, u[3];
     tb_collide<L, St, OP>(a, n, f, fs, rho, u);
-    sfor<Q>([&](auto J) {
-      constexpr int j = decltype(J)::value;
-      ring[slot][j][t] = static_cast<S>(fs[j]);
-    });
+    sfor<Q>([&](auto J) { ring[at(J, it, t)] = static_cast<S>(fs[decltype(J)::value]); });
   };
-  // second step on plane p2 of the core; s1 = its slot, sm / sp = the slots
-  // of planes p2 - 1 / p2 + 1
+  // this thread's node of the second step (core column) and its class word
+  const int ccx = t % TX, ccy = t / TX;
+  const bool core_ok = t < TX * TY && ccx < wx && ccy < wy;
+  const uint32_t core_xy = (uint32_t)(x0 + ccx) + (uint32_t)nx * (uint32_t)(y0 + ccy);
+  auto core_class = [&](int p) -> uint32_t {
+    return core_ok ? a.cls[core_xy + (uint32_t)p * nxy] : kSolidBit;
+  };
+  // second step in march iteration `it` on plane p2 of the core, whose
+  // first-step values the iteration before wrote (it - 2: the plane below,
+  // it: the plane above)
   T* rho_c = (T*)a.rho[par ^ 1];
   T* u_c = (T*)a.u[par ^ 1];
-  auto second = [&](int p2, int sm, int s1, int sp) {
-    if (t >= TX * TY) return;
-    const int cx = t % TX, cy = t / TX;
-    if (cx >= wx || cy >= wy) return;
-    const int h = (cy + 1) * HX + cx + 1;
-    const uint32_t cls = cring[s1][h];
+  auto second = [&](int p2, int it, uint32_t cls) {
     if (cls & kSolidBit) return;
+    const int cx = ccx, cy = ccy;
+    const int h = (cy + 1) * HX + cx + 1;
     const int x = x0 + cx, y = y0 + cy;
     const uint32_t on = ((x == 0 ? 1u : 0u) | (x == nx - 1 ? 2u : 0u) | (y == 0 ? 4u : 0u) |
                          (y == ny - 1 ? 8u : 0u) | (p2 == 0 ? 16u : 0u) |
                          (p2 == nz - 1 ? 32u : 0u)) & act;
     const uint32_t sel = (cls & kLinkMask) | face_mask<L>(on);
     T f[Q], fs[Q], rho, u[3];
-    const S* flat = &ring[0][0][0];
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;
       constexpr int jb = k::qbar(j);
       constexpr int cz = k::cz(j);
-      const int sl = cz > 0 ? sm : (cz < 0 ? sp : s1);
+      const int src = cz > 0 ? it - 2 : (cz < 0 ? it : it - 1);
       // one LDS read per direction: the address selects pull or bounce-back
-      const int pulled = (sl * Q + j) * HN + h - k::cx(j) - k::cy(j) * HX;
-      const int back = (s1 * Q + jb) * HN + h;
-      f[j] = T(flat[(sel >> jb & 1u) ? back : pulled]);
+      const int pulled = at(J, src, h - k::cx(j) - k::cy(j) * HX);
+      const int back = at(std::integral_constant<int, jb>{}, it - 1, h);
+      f[j] = T(ring[(sel >> jb & 1u) ? back : pulled]);
     });
-    if (on)
-      tb_faces<L, T>(a, on, f, [&](auto I) { return T(ring[s1][decltype(I)::value][h]); });
+    if (on && !a.faces_noslip_only)
+      tb_faces<L, T>(a, on, f, [&](auto I) { return T(ring[at(I, it - 1, h)]); });
     const uint32_t n = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)p2);
     tb_collide<L, St, OP>(a, n, f, fs, rho, u);
     if constexpr (MOM) {
@@ -305,48 +353,43 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
 
   // ---- march: iteration i runs the first step on plane z0 - 1 + i and the
   // second step on plane z0 - 2 + i (from i = 2). The next plane's loads go
-  // out after the first barrier (kEarly: before the first step's arithmetic,
-  // which measured 5 % slower in FP32 and spills in FP64).
-  constexpr bool kEarly = false;
+  // out after the barrier (issuing them before the first step's arithmetic
+  // measured 5 % slower in FP32 and spills in FP64).
   const int np = (z1 - z0) + 2;
   uint32_t c_next = class_of(z0);
+  uint32_t c2_next = core_class(z0);  // the first second step's plane
   issue(z0 - 1, class_of(z0 - 1));
   for (int i = 0; i < np; i++) {
-    const int slot = i % 3;
     // take this plane's loaded values (post-stream f after the faces)
     auto take = [&](T* f) {
       if (v_cls & kSolidBit) return;
       sfor<Q>([&](auto J) { f[decltype(J)::value] = T(v[decltype(J)::value]); });
-      // the gather loaded f*_i into v[ibar] on every face-crossing link i
-      if (!fresh && v_on)
+      // the gather loaded f*_i into v[ibar] on every face-crossing link i,
+      // which is all a no-slip face asks
+      if (!fresh && v_on && !a.faces_noslip_only)
         tb_faces<L, T>(a, v_on, f, [&](auto I) { return T(v[k::qbar(decltype(I)::value)]); });
     };
-    auto next = [&]() {
-      if (i + 1 < np) {
-        const uint32_t c = c_next;
-        if (i + 2 < np) c_next = class_of(z0 + i + 1);
-        issue(z0 + i, c);
-      }
-    };
-    if constexpr (kEarly) {
-      // the next plane's loads are in flight across this plane's first step,
-      // the barriers and the second step
+    [&]() {
       T f[Q];
-      const uint32_t cls = v_cls, n1 = v_n;
       take(f);
-      next();
-      first(slot, f, cls, n1);
-    } else {
-      [&]() {
-        T f[Q];
-        take(f);
-        first(slot, f, v_cls, v_n);
-      }();
+      first(i, f, v_cls, v_n);
+    }();
+    // the only barrier of the iteration: this plane's ring slot is complete,
+    // and every wave has left the previous iteration's second step, whose
+    // ring slots the next first step may overwrite
+    __syncthreads();
+    uint32_t c2 = kSolidBit;
+    if (i >= 2) {
+      c2 = c2_next;
+      if (i + 1 < np) c2_next = core_class(z0 - 1 + i);
     }
-    __syncthreads();  // ring slot `slot` complete
-    if constexpr (!kEarly) next();
-    if (i >= 2) second(z0 - 2 + i, (i + 1) % 3, (i + 2) % 3, slot);
-    __syncthreads();  // second step's reads done before the next first step overwrites
+    // the next plane's loads, in flight across the second step
+    if (i + 1 < np) {
+      const uint32_t c = c_next;
+      if (i + 2 < np) c_next = class_of(z0 + i + 1);
+      issue(z0 + i, c);
+    }
+    if (i >= 2) second(z0 - 2 + i, i, c2);
   }
 }
 
