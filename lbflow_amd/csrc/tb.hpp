@@ -181,7 +181,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
   using S = typename St::S;
   using k = K<L>;
   using Sh = TbShape<L, St, SHAPE>;
-  constexpr int Q = L::Q, TX = Sh::TX, TY = Sh::TY, HX = Sh::HX, HN = Sh::HN;
+  constexpr int Q = L::Q, TX = Sh::TX, TY = Sh::TY, HX = Sh::HX, HY = Sh::HY, HN = Sh::HN;
   static_assert(L::D == 3, "two-step passes march along z");
   constexpr int NUP = Sh::NUP, NREST = Sh::NREST;
   // ring of the c_z = +1 directions (4 planes), then the others (3 planes)
@@ -221,9 +221,19 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
   const int z0 = seg * a.tb_zs, z1 = min(z0 + a.tb_zs, nz);
   if (z0 >= nz) return;  // whole workgroup: no barrier is skipped by a part of it
 
-  // ---- this thread's node of the first step's ring plane
+  // ---- this thread's node of the first step's ring plane: threads
+  // [0, TX * HY) take the tile's own columns of the ring rows, one row of TX
+  // nodes after another (a wavefront's loads then cover whole rows), the
+  // next 2 HY threads the ring's two side columns
   const int t = threadIdx.x;
-  const int lx = t % HX, ly = t / HX;
+  int lx, ly;
+  if (t < TX * HY) {
+    ly = t / TX, lx = 1 + t % TX;
+  } else {
+    const int e = t - TX * HY;
+    ly = e >> 1, lx = (e & 1) ? TX + 1 : 0;
+  }
+  const int hh = ly * HX + lx;  // ring index of that node
   bool h_ok = t < HN && lx <= wx + 1 && ly <= wy + 1;
   int gx = x0 - 1 + lx, gy = y0 - 1 + ly;
   if (gx < 0 || gx >= nx) {
@@ -299,7 +309,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
     if (cls & kSolidBit) return;
     T fs[Q], rho, u[3];
     tb_collide<L, St, OP>(a, n, f, fs, rho, u);
-    sfor<Q>([&](auto J) { ring[at(J, it, t)] = static_cast<S>(fs[decltype(J)::value]); });
+    sfor<Q>([&](auto J) { ring[at(J, it, hh)] = static_cast<S>(fs[decltype(J)::value]); });
   };
   // this thread's node of the second step (core column) and its class word
   const int ccx = t % TX, ccy = t / TX;
