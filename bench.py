@@ -5,15 +5,17 @@ porous solid, 256^3 nodes, periodic in x, no-slip y/z walls, force 1e-6
 along x (magic parameter 3/16, tau 0.8), populations in FP32. One "step" is
 one lattice time step of the whole 256^3 lattice. MLUPS = N * steps / t / 1e6.
 
-Prints one JSON line. Besides the headline (FP32), the same workload is
-timed under "extra" in FP64 (f64_aa: the default FP64 path, in place, with
-the reference's bits; f64: the two-array scheme), FP32 on two arrays and the
-shifted FP32/FP16/BF16 storage tiers. The roofline block uses algorithmic
+Prints one JSON line. The headline (FP32) runs two time steps per launch
+(LBF_STREAM_TWO_STEP). Besides it, the same workload is timed under "extra"
+in FP32 in place (f32_aa, the same bits), FP64 (f64_aa: the default FP64
+path, in place, with the reference's bits; f64: the two-array scheme;
+f64_two_step), FP32 on two arrays and the shifted FP32/FP16/BF16 storage
+tiers. The roofline block uses algorithmic
 bytes 2*Q*s per node update; the CPU baseline times the oracle (the
 line-by-line f64 restatement of the reference solver) on the host cores for
-a bounded sample of the same workload. The same kernel on an all-fluid, fully
+a bounded sample of the same workload. The same kernels on an all-fluid, fully
 periodic 256^3 box (uniform initial velocity 0.05 along x, no bounce-back
-links) is reported as extra.all_fluid_periodic: the clean roofline point.
+links) are reported as extra.all_fluid_periodic: the clean roofline point.
 
 drop_in_loop: the headline workload through lbf_run, the reference's time loop
 with residual norms and the stop rule after every step (src/momentum.rs:273-292).
@@ -637,9 +639,11 @@ def main():
     # so this isolates the streaming pattern from the porous geometry
     all_fluid = {}
     if not args.no_all_fluid:
-        for name, prec, sbytes in (("f32", abi.F32, 4), ("f32s", abi.F32_SHIFTED, 4),
-                                   ("f16s", abi.F16_SHIFTED, 2)):
-            sp = configs.periodic_box(n=n, precision=prec, streaming=abi.STREAM_AA).replace(
+        for name, prec, sbytes, streaming in (
+                ("f32", abi.F32, 4, abi.STREAM_TWO_STEP), ("f32_aa", abi.F32, 4, abi.STREAM_AA),
+                ("f32s", abi.F32_SHIFTED, 4, abi.STREAM_AA),
+                ("f16s", abi.F16_SHIFTED, 2, abi.STREAM_AA)):
+            sp = configs.periodic_box(n=n, precision=prec, streaming=streaming).replace(
                 initial_velocity=(0.05, 0.0, 0.0))
             ms, wall, nonfinite, mass = time_case(sp, args.steps, args.warmup, hip)
             mlups = N * args.steps / (ms / 1e3) / 1e6
