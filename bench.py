@@ -608,7 +608,10 @@ def main():
                 ("f32s", abi.F32_SHIFTED, abi.STREAM_AA, 4),
                 ("f32_two_array", abi.F32, abi.STREAM_TWO_ARRAY, 4),
                 ("f64_aa", abi.F64, abi.STREAM_AA, 8),
-                ("f64_two_step", abi.F64, abi.STREAM_TWO_STEP, 8)]
+                ("f64_two_step", abi.F64, abi.STREAM_TWO_STEP, 8),
+                ("f16s_two_step", abi.F16_SHIFTED, abi.STREAM_TWO_STEP, 2),
+                ("bf16s_two_step", abi.BF16_SHIFTED, abi.STREAM_TWO_STEP, 2),
+                ("f32s_two_step", abi.F32_SHIFTED, abi.STREAM_TWO_STEP, 4)]
     if args.only:
         by_name = {v[0]: v for v in variants}
         variants = [by_name[k] for k in args.only.split(",")]

@@ -112,7 +112,8 @@ extern "C" {
                                   first step's values held in LDS, where the
                                   steps need no residuals; other steps as
                                   LBF_STREAM_TWO_ARRAY, bit for bit the same
-                                  (3-D lattices, FP64 / FP32, faces no-slip,
+                                  (shifted storage: as LBF_STREAM_AA)
+                                  (3-D lattices, any storage, faces no-slip,
                                   bounce-back or periodic, no scalars)        */
 
 /* ---- source kinds (force / scalar source) ------------------------------- */

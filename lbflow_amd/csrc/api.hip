@@ -119,12 +119,12 @@ bool aa_reproduces_reference(const lbf_case_desc* d, const VsInfo& v) {
   return true;
 }
 
-// Two steps per pass (tb.hpp): 3-D lattices in FP64 / FP32 storage, faces
-// no-slip, moving bounce-back or periodic (opposite faces alike), no scalars
-// and no Shan-Chen force.
+// Two steps per pass (tb.hpp): 3-D lattices in any storage, faces no-slip,
+// moving bounce-back or periodic (opposite faces alike), no scalars and no
+// Shan-Chen force.
 bool tb_compatible(const lbf_case_desc* d, const VsInfo& v) {
   if (v.d != 3 || d->n_scalars || d->multiphase) return false;
-  if (d->precision != LBF_F64 && d->precision != LBF_F32) return false;
+  if (d->precision < LBF_F64 || d->precision > LBF_BF16_SHIFTED) return false;
   for (int f = 0; f < 2 * v.d; f++) {
     const int k = d->faces[f].kind;
     if (k != LBF_BC_NO_SLIP && k != LBF_BC_BOUNCE_BACK && k != LBF_BC_PERIODIC) return false;
@@ -239,6 +239,12 @@ struct lbf_case {
   // (their launches carry it as step_offset; flush_steps advances the counter)
   uint32_t pending_steps = 0;
   bool aa_fresh = false;   // AA: array holds post-stream f (after init / set_pops)
+  // AA: the in-place pattern of step t is odd when (t + aa_shift) is odd. A
+  // fresh array (post-stream f in natural slots) must start with the even
+  // pattern, which reads and writes only the node's own slots; the odd
+  // pattern writes the neighbours' slots, which they would still be reading
+  // as their own fresh values. initialize_nodes and set_pops set the shift.
+  int aa_shift = 0;
   int xcd_group = 0;       // AA: runs of consecutive blocks per XCD (0: round-robin)
   bool may_have_stopped = false;  // steps were enqueued with the stop rule on
   double scale = 1.0;    // shifted storage: stored = (f - w) * scale
@@ -484,7 +490,9 @@ int lbf_device_bytes(const lbf_case_desc* d, uint64_t* bytes) {
   const long long ld = plane_stride(N);
   const size_t sb = storage_bytes(d->precision), tb = d->precision == LBF_F64 ? 8 : 4;
   const bool aa_ok = aa_compatible(d, v);
-  const bool forced = d->precision >= LBF_F32_SHIFTED || d->streaming == LBF_STREAM_AA;
+  const bool forced =
+      (d->precision >= LBF_F32_SHIFTED && d->streaming != LBF_STREAM_TWO_STEP) ||
+      d->streaming == LBF_STREAM_AA;
   bool aa = forced || (d->streaming == LBF_STREAM_AUTO && aa_ok &&
                        (d->precision != LBF_F64 || aa_reproduces_reference(d, v)));
   // as lbf_create: the automatic choice falls back to two arrays at 4 GiB
@@ -530,24 +538,23 @@ int lbf_create(const lbf_case_desc* d, lbf_case** out) {
     }
     if (shifted && !aa_ok) {
       delete c;
-      return fail(LBF_EUNSUPPORTED, "shifted storage runs on the AA kernel only (no scalars, "
-                                    "no Shan-Chen force, faces in {no-slip, bounce-back, "
-                                    "periodic, Zou-He})");
+      return fail(LBF_EUNSUPPORTED, "shifted storage runs on the AA or two-step kernels only "
+                                    "(no scalars, no Shan-Chen force, faces in {no-slip, "
+                                    "bounce-back, periodic, Zou-He})");
     }
     if (d->streaming == LBF_STREAM_TWO_STEP) {
       if (!tb_compatible(d, v)) {
         delete c;
-        return fail(LBF_EUNSUPPORTED, "two-step passes need a 3-D lattice, FP64 or FP32 "
-                                      "storage, no scalars, no Shan-Chen force and faces in "
-                                      "{no-slip, bounce-back, periodic} with matching "
-                                      "periodicity of opposite faces");
+        return fail(LBF_EUNSUPPORTED, "two-step passes need a 3-D lattice, no scalars, no "
+                                    "Shan-Chen force and faces in {no-slip, bounce-back, "
+                                    "periodic} with matching periodicity of opposite faces");
       }
       c->two_step = true;
     }
     if (d->streaming == LBF_STREAM_AA) c->aa = true;
     else if (d->streaming == LBF_STREAM_AUTO)
       c->aa = aa_ok && (c->prec != LBF_F64 || aa_reproduces_reference(d, v));
-    if (shifted) c->aa = true;
+    if (shifted && !c->two_step) c->aa = true;
     if (c->aa && (unsigned long long)c->ld * c->q * c->sb >= (1ull << 32)) {
       if (shifted || d->streaming == LBF_STREAM_AA) {
         delete c;
@@ -794,6 +801,9 @@ void lbf_destroy(lbf_case* c) {
 
 namespace {
 
+// in-place pattern of the next step: 1 = odd
+int aa_phase(const lbf_case* c) { return (int)((c->host_step + (uint64_t)c->aa_shift) & 1); }
+
 KArgs momentum_args(lbf_case* c) {
   KArgs a = c->base;
   a.field = 0;
@@ -805,7 +815,7 @@ KArgs momentum_args(lbf_case* c) {
   a.scale = c->scale;
   a.plane_bytes = (uint32_t)(c->ld * c->sb);
   a.fresh = c->aa_fresh ? 1 : 0;
-  a.aa_odd = (int)(c->host_step & 1);
+  a.aa_odd = aa_phase(c);
   a.xcd_group = c->aa ? c->xcd_group : 0;
   a.tb_zs = c->ts_zs;
   a.tb_shape = c->ts_shape;
@@ -987,7 +997,7 @@ int enqueue_steps(lbf_case* c, uint64_t nsteps, uint32_t flags, bool every_last)
       k += 2;
       continue;
     }
-    if (c->graphs_on && !c->aa_fresh && (c->host_step & 1) == 0 &&
+    if (c->graphs_on && !c->aa_fresh && aa_phase(c) == 0 &&
         (every_last ? left >= kGraphSteps : left > kGraphSteps)) {
       hipGraphExec_t g = nullptr;
       int rc = flush_steps(c);  // the graph's launches count from the device counter
@@ -1011,6 +1021,14 @@ extern "C" {
 
 int lbf_initialize_nodes(lbf_case* c) {
   if (!c) return fail(LBF_EINVAL, "case is NULL");
+  if (c->may_have_stopped) {
+    // the host step count may be ahead of a run that stopped early
+    Ctrl h;
+    int rc = read_ctrl(c, &h);
+    if (rc) return rc;
+    c->may_have_stopped = false;
+  }
+  c->aa_shift = (int)(c->host_step & 1);  // the fresh step takes the even pattern
   KArgs a = momentum_args(c);
   if (c->aa)
     HIPCHK(c->L.aa_init(a, c->grid, c->stream));
@@ -1314,7 +1332,10 @@ int lbf_set_pops(lbf_case* c, int32_t field, const double* in) {
   }
   HIPCHK(copy_sync(c, dst, buf.data(), buf.size(), hipMemcpyHostToDevice));
   HIPCHK(launch_ctrl_set(c->ctrl, kCtrlFresh, 1ll << field, c->stream));
-  if (field == 0) c->aa_fresh = true;
+  if (field == 0) {
+    c->aa_fresh = true;
+    c->aa_shift = (int)(c->host_step & 1);  // the fresh step takes the even pattern
+  }
   return 0;
 }
 
@@ -1401,6 +1422,9 @@ int set_error(int code, const char* msg) { return fail(code, "%s", msg); }
 
 int case_serve_info(lbf_case* c, CaseServeInfo* out) {
   if (!c || !out) return fail(LBF_EINVAL, "NULL argument");
+  // the server derives each case's in-place pattern from its step count
+  if (c->aa_shift)
+    return fail(LBF_EUNSUPPORTED, "case restarted at an odd step: its in-place pattern is shifted");
   out->velocity_set = c->desc.velocity_set;
   out->precision = c->prec;
   out->op = c->desc.collision.op;

@@ -231,7 +231,9 @@ __device__ __forceinline__ void apply_momentum_faces(const KArgs& a, uint32_t on
           f[ib] = fsi;  // bc.rs:119-129
         } else if (bc.kind == 2) {
           const T udc = udotc<L, i>(vel);  // bc.rs:62-84
-          f[ib] = fsi - T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+          // shifted storage (f - w) * scale: the correction scales, w_ib = w_i
+          const T dv = T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+          f[ib] = fsi - (St::shifted ? T(a.scale) * dv : dv);
         } else {
           const T udc = udotc<L, i>(vel);  // bc.rs:86-117
           f[ib] = -fsi + T(2.0) * T(k::w(i)) * T(bc.density) *
@@ -361,7 +363,16 @@ __global__ void __launch_bounds__(kBlock) momentum_step_plain(KArgs a) {
       if (on) apply_momentum_faces<L, St>(a, on, f, src, n, g);
     }
     if (a.mode == kModeStreamOnly) {
-      readback_store<Q>(a, n, f);
+      if constexpr (St::shifted) {
+        double fd[Q];
+        sfor<Q>([&](auto J) {
+          constexpr int j = decltype(J)::value;
+          fd[j] = (double)f[j] * a.inv_scale + k::w(j);
+        });
+        readback_store<Q>(a, n, fd);
+      } else {
+        readback_store<Q>(a, n, f);
+      }
       continue;
     }
     // FP32 storage collides in pair form (pair_collide, shared with the
@@ -782,7 +793,13 @@ __global__ void __launch_bounds__(kBlock) momentum_init(KArgs a) {
         constexpr int ax = decltype(A)::value;
         u[ax] = uu[ax * a.ld + n];
       });
-      equilibrium<L>(rho[n], u, feq);
+      if constexpr (St::shifted) {
+        // stored units scale * (f_eq - w)
+        equilibrium_shifted<L>(rho[n] - T(1), rho[n], u, feq);
+        sfor<Q>([&](auto J) { feq[decltype(J)::value] *= T(a.scale); });
+      } else {
+        equilibrium<L>(rho[n], u, feq);
+      }
     }
     sfor<Q>([&](auto J) {
       constexpr int j = decltype(J)::value;

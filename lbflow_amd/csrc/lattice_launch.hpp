@@ -144,18 +144,30 @@ static bool make_launchers(int precision, Launchers* out) {
       out->aa_step = aa_step_t<L, StF32S>;
       out->aa_init = aa_init_t<L, StF32S>;
       out->aa_readback = aa_readback_t<L, StF32S>;
+      // two arrays with shifted storage: the two-step scheme and its single steps
+      out->mom_step = mom_step_plain<L, StF32S>;
+      out->mom_init = mom_init_t<L, StF32S>;
+      tb_launchers<L, StF32S>(out);
       serve_launchers<L, StF32S>(out);
       return true;
     case 3:  // LBF_F16_SHIFTED
       out->aa_step = aa_step_t<L, StF16S>;
       out->aa_init = aa_init_t<L, StF16S>;
       out->aa_readback = aa_readback_t<L, StF16S>;
+      // two arrays with shifted storage: the two-step scheme and its single steps
+      out->mom_step = mom_step_plain<L, StF16S>;
+      out->mom_init = mom_init_t<L, StF16S>;
+      tb_launchers<L, StF16S>(out);
       serve_launchers<L, StF16S>(out);
       return true;
     case 4:  // LBF_BF16_SHIFTED
       out->aa_step = aa_step_t<L, StBF16S>;
       out->aa_init = aa_init_t<L, StBF16S>;
       out->aa_readback = aa_readback_t<L, StBF16S>;
+      // two arrays with shifted storage: the two-step scheme and its single steps
+      out->mom_step = mom_step_plain<L, StBF16S>;
+      out->mom_init = mom_init_t<L, StBF16S>;
+      tb_launchers<L, StBF16S>(out);
       serve_launchers<L, StBF16S>(out);
       return true;
     default:

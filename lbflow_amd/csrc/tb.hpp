@@ -16,7 +16,7 @@
 // reference's order without contraction; FP32: pair_collide), with the same
 // pull / bounce-back / face order, so one pass gives the bits of two
 // momentum_step_plain launches in either storage.
-// Scope: 3-D lattices, FP64 / FP32 storage, faces periodic, no-slip or moving
+// Scope: 3-D lattices, FP64 / FP32 / shifted storage, faces periodic, no-slip or moving
 // bounce-back (opposite faces both periodic or both not), no scalars, no
 // Shan-Chen force, no residuals (those steps run the single-step kernel).
 #pragma once
@@ -85,7 +85,7 @@ struct TbShape {
 // order, as apply_momentum_faces; own(i) is the node's own f*_i. The gather
 // already put f_ibar = f*_i on every link that crosses a non-periodic face.
 // own is called with std::integral_constant<int, i>.
-template <class L, class T, class Own>
+template <class L, class T, bool SHIFTED, class Own>
 __device__ __forceinline__ void tb_faces(const KArgs& a, uint32_t on, T* f, Own&& own) {
   using k = K<L>;
   using Dv = Derived<L>;
@@ -104,7 +104,9 @@ __device__ __forceinline__ void tb_faces(const KArgs& a, uint32_t on, T* f, Own&
         f[ib] = fsi;  // bc.rs:119-129
       } else if (bc.kind == 2) {
         const T udc = udotc<L, i>(vel);  // bc.rs:62-84
-        f[ib] = fsi - T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+        // shifted storage (f - w) * scale: the correction scales, w_ib = w_i
+        const T dv = T(2.0) * T(k::w(i)) * T(bc.density) * T(3.0) * udc;
+        f[ib] = fsi - (SHIFTED ? T(a.scale) * dv : dv);
       }
     });
   });
@@ -344,7 +346,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
       f[j] = T(ring[(sel >> jb & 1u) ? back : pulled]);
     });
     if (on && !a.faces_noslip_only)
-      tb_faces<L, T>(a, on, f, [&](auto I) { return T(ring[at(I, it - 1, h)]); });
+      tb_faces<L, T, St::shifted>(a, on, f, [&](auto I) { return T(ring[at(I, it - 1, h)]); });
     const uint32_t n = (uint32_t)x + (uint32_t)nx * ((uint32_t)y + (uint32_t)ny * (uint32_t)p2);
     tb_collide<L, St, OP>(a, n, f, fs, rho, u);
     if constexpr (MOM) {
@@ -366,6 +368,10 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
   // out after the barrier (issuing them before the first step's arithmetic
   // measured 5 % slower in FP32 and spills in FP64).
   const int np = (z1 - z0) + 2;
+#ifdef LBF_TB_POISON
+  for (int e = t; e < (int)(sizeof(ring) / sizeof(S)); e += Sh::NT) ring[e] = S(__builtin_nanf(""));
+  __syncthreads();
+#endif
   uint32_t c_next = class_of(z0);
   uint32_t c2_next = core_class(z0);  // the first second step's plane
   issue(z0 - 1, class_of(z0 - 1));
@@ -377,7 +383,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
       // the gather loaded f*_i into v[ibar] on every face-crossing link i,
       // which is all a no-slip face asks
       if (!fresh && v_on && !a.faces_noslip_only)
-        tb_faces<L, T>(a, v_on, f, [&](auto I) { return T(v[k::qbar(decltype(I)::value)]); });
+        tb_faces<L, T, St::shifted>(a, v_on, f, [&](auto I) { return T(v[k::qbar(decltype(I)::value)]); });
     };
     [&]() {
       T f[Q];

@@ -8,6 +8,7 @@ step's post-collision values in LDS, so the bar is bit for bit:
   - in FP64 also equal to the oracle (the two-array FP64 path's own bar);
   - in FP32 also equal to the in-place (AA) kernel: all three FP32 schemes
     collide through the same pair-form code (pair_collide.inc);
+  - in shifted storage (FP32 / FP16 / BF16) equal to the in-place kernel;
 over even and odd step counts (an odd count ends with a single step), split
 batches, batches with residual steps in between (single steps), lbf_run, a
 restart from lbf_set_pops (the pass then starts from post-stream values),
@@ -172,9 +173,60 @@ def test_z_segments(zs, monkeypatch):
     assert_same(a, b)
 
 
+@pytest.mark.parametrize("precision", [abi.F32_SHIFTED, abi.F16_SHIFTED, abi.BF16_SHIFTED])
+@pytest.mark.parametrize("name", list(CASES))
+def test_shifted_passes_equal_in_place(name, precision):
+    # shifted storage has no plain two-array scheme to compare with: the
+    # in-place kernel runs the same pair-form arithmetic on the same stored values
+    print(f"[two-step-shifted] {name} precision={precision}", flush=True)
+    sp = CASES[name].replace(precision=precision)
+    a = Case(sp.replace(streaming=abi.STREAM_TWO_STEP))
+    b = Case(sp.replace(streaming=abi.STREAM_AA))
+    for c in (a, b):
+        c.initialize_nodes()
+    for n in (8, 3, 5, 2):
+        a.main_steps(n)
+        b.main_steps(n)
+        assert_same(a, b)
+    a.main_steps(3, residuals=True)
+    b.main_steps(3, residuals=True)
+    assert_same(a, b)
+    f = a.pops()
+    a.set_pops(f)
+    b.set_pops(f)
+    a.main_steps(4)
+    b.main_steps(4)
+    assert_same(a, b)
+
+
+@pytest.mark.parametrize("precision", [abi.F32, abi.F16_SHIFTED])
+def test_in_place_restart_at_odd_step(precision):
+    # a restart after an odd step count: the in-place kernel starts its
+    # pattern with the even (own-slot) form and gives the two-step kernel's
+    # bits, on every repetition
+    print(f"[two-step-restart-odd] precision={precision}", flush=True)
+    sp = CASES["porous_trt_force"].replace(precision=precision)
+    t = Case(sp.replace(streaming=abi.STREAM_TWO_STEP))
+    t.initialize_nodes()
+    t.main_steps(7)
+    f = t.pops()
+    t.set_pops(f)
+    t.main_steps(6)
+    for _ in range(3):
+        a = Case(sp.replace(streaming=abi.STREAM_AA))
+        a.initialize_nodes()
+        a.main_steps(7)
+        a.set_pops(f)
+        a.main_steps(6)
+        assert_same(a, t)
+        a.close()
+
+
 def test_unsupported_cases_are_refused():
+    anti = configs.porous_channel(n=(12, 10, 8))
+    anti.faces = anti.faces[:4] + [FaceBC.anti_bounce_back(1.01), FaceBC.anti_bounce_back(1.0)]
     for sp in (configs.cavity(n=16),                                     # 2-D
-               configs.porous_channel(n=(12, 10, 8), precision=abi.F16_SHIFTED),
+               anti,                                                     # anti-bounce-back
                configs.sphere_channel(nx=20, ny=10, nz=12, radius=3)):  # Zou-He faces
         with pytest.raises(LbfError) as e:
             Case(sp.replace(streaming=abi.STREAM_TWO_STEP))
