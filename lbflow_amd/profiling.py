@@ -40,7 +40,8 @@ def variant_of(kernel_name):
     if scheme == "plain":
         return "f64" if v == "f64" else v + "_two_array"
     if scheme == "tb2":
-        return "f64_two_step" if v == "f64" else v
+        # the FP32 two-step kernel is the headline variant "f32"
+        return "f32" if v == "f32" else v + "_two_step"
     if v == "f64":
         return "f64_aa"
     return "f32_aa" if v == "f32" else v
@@ -104,7 +105,8 @@ def traffic_summary(fetch_csv, write_csv, copy_bytes, cmd_fetch, cmd_write, node
     for k, (rb, wb, cnt) in per.items():
         v = variant_of(k)
         s = {"f32": 4, "f32_aa": 4, "f16s": 2, "bf16s": 2, "f32s": 4, "f64": 8, "f64_aa": 8,
-             "f32_two_array": 4, "f64_two_step": 8}.get(v)
+             "f32_two_array": 4, "f64_two_step": 8, "f16s_two_step": 2, "bf16s_two_step": 2,
+             "f32s_two_step": 4}.get(v)
         # a two-step launch (momentum_step_tb2) advances two time steps
         steps = 2 if "momentum_step_tb2" in k else 1
         alg = 2.0 * q * nodes * s * steps if s else None
