@@ -173,6 +173,39 @@ def test_z_segments(zs, monkeypatch):
     assert_same(a, b)
 
 
+def tiny_cases():
+    rng = np.random.default_rng(17)
+    m3 = np.zeros((3, 2, 2), np.uint8)
+    m3[1, 0, 1] = 1
+    box = Spec(velocity_set=abi.D3Q19, n=(5, 4, 6), collision=Collision.trt(1.1, 0.9),
+               faces=[FaceBC.no_slip()] * 5 + [FaceBC.bounce_back(1.0, (0.04, 0.0, 0.0))],
+               initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0))
+    return {
+        # one-node-wide and two-node axes wrap onto themselves
+        "d3q19_2x2x3_solid": configs.periodic_box(n=(2, 2, 3)).replace(
+            node_type=m3.reshape(-1), initial_velocity=rng.uniform(-0.02, 0.02, (12, 3))),
+        "d3q27_1x1x4": configs.periodic_box(n=(1, 1, 4), velocity_set=abi.D3Q27).replace(
+            initial_velocity=rng.uniform(-0.02, 0.02, (4, 3))),
+        "d3q19_nz1": configs.periodic_box(n=(9, 7, 1)).replace(
+            initial_velocity=rng.uniform(-0.02, 0.02, (63, 3))),
+        "d3q15_nz2_walls": configs.porous_channel(n=(11, 6, 2), velocity_set=abi.D3Q15,
+                                                  force=1e-4, seed=3),
+        # every face non-periodic, a moving lid on top
+        "closed_box_lid": box,
+    }
+
+
+@pytest.mark.parametrize("precision", [abi.F64, abi.F32])
+@pytest.mark.parametrize("name", list(tiny_cases()))
+def test_tiny_lattices(name, precision):
+    print(f"[two-step-tiny] {name} precision={precision}", flush=True)
+    a, b = pair(tiny_cases()[name], precision)
+    for n in (6, 3, 4):
+        a.main_steps(n)
+        b.main_steps(n)
+        assert_same(a, b)
+
+
 @pytest.mark.parametrize("precision", [abi.F32_SHIFTED, abi.F16_SHIFTED, abi.BF16_SHIFTED])
 @pytest.mark.parametrize("name", list(CASES))
 def test_shifted_passes_equal_in_place(name, precision):
