@@ -5,20 +5,20 @@
 // the z axis and marches along z. For each plane p = z0-1 .. z1 it runs the
 // first step (pull from the source array, bounce-back, faces, collide) on the
 // tile plus a one-node ring around it and keeps the post-collision values in
-// an LDS ring of three planes. It then runs the second step on plane p-1 of
-// the tile itself, pulling every population from LDS, and stores f* to the
+// an LDS ring of planes. It then runs the second step on plane p-1 of the
+// tile itself, pulling every population from LDS, and stores f* to the
 // destination array. Per two steps a node's Q values are read about
 // (1 + ring share) times and written once, where two single steps read and
 // write them twice each; neighbouring tiles share their ring reads through
 // L2 (tiles of one z segment run on one XCD, see tb_tile).
 //
 // Arithmetic: the two-array kernel's (FP64: node_math.hpp calls in the
-// reference's order without contraction; FP32: pair_collide), with the same
-// pull / bounce-back / face order, so one pass gives the bits of two
-// momentum_step_plain launches in either storage.
-// Scope: 3-D lattices, FP64 / FP32 / shifted storage, faces periodic, no-slip or moving
-// bounce-back (opposite faces both periodic or both not), no scalars, no
-// Shan-Chen force, no residuals (those steps run the single-step kernel).
+// reference's order without contraction; FP32 and shifted storage:
+// pair_collide), with the same pull / bounce-back / face order, so one pass
+// gives the bits of two momentum_step_plain launches in every storage.
+// Scope: 3-D lattices, faces periodic, no-slip or moving bounce-back
+// (opposite faces both periodic or both not), no scalars, no Shan-Chen
+// force, no residuals (those steps run the single-step kernel).
 #pragma once
 
 #include "fast.hpp"
@@ -114,7 +114,8 @@ __device__ __forceinline__ void tb_faces(const KArgs& a, uint32_t on, T* f, Own&
 
 // moments, force, collision of one node: the two-array kernel's sequence
 // (kernels.hpp momentum_step_plain) for a node without Shan-Chen force;
-// FP32 in pair form (pair_collide), FP64 in the reference's order
+// FP32 and shifted storage in pair form (pair_collide), FP64 in the
+// reference's order
 template <class L, class St, int OP>
 __device__ __forceinline__ void tb_collide(const KArgs& a, uint32_t n, const typename St::T* f,
                                            typename St::T* fs, typename St::T& rho,
@@ -389,6 +390,8 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(1, TbShape<L, St, SHA
   // measured 5 % slower in FP32 and spills in FP64).
   const int np = (z1 - z0) + 2;
 #ifdef LBF_TB_POISON
+  // check builds: the ring starts as NaN, so a read of a slot that no first
+  // step of this launch wrote shows in the results
   for (int e = t; e < (int)(sizeof(ring) / sizeof(S)); e += Sh::NT) ring[e] = S(__builtin_nanf(""));
   __syncthreads();
 #endif
