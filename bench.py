@@ -16,6 +16,9 @@ line-by-line f64 restatement of the reference solver) on the host cores for
 a bounded sample of the same workload. The same kernels on an all-fluid, fully
 periodic 256^3 box (uniform initial velocity 0.05 along x, no bounce-back
 links) are reported as extra.all_fluid_periodic: the clean roofline point.
+Every achieved_gbs / hbm_frac counts those algorithmic bytes, so a two-step
+variant, which moves fewer real bytes than two single steps, can pass 1.0;
+roofline.traffic holds the HBM bytes the headline kernel really moves.
 
 drop_in_loop: the headline workload through lbf_run, the reference's time loop
 with residual norms and the stop rule after every step (src/momentum.rs:273-292).
@@ -636,8 +639,8 @@ def main():
                              frac_of_sweep=round(gbs / sweep_gbs[sbytes], 4),
                              streaming={abi.STREAM_AA: "aa", abi.STREAM_TWO_ARRAY: "two-array",
                                         abi.STREAM_TWO_STEP: "two-step"}[streaming])
-        print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)",
-              file=sys.stderr, flush=True)
+        print(f"[bench] {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s algorithmic "
+              f"({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)", file=sys.stderr, flush=True)
     # the same kernel on an all-fluid, fully periodic box: no bounce-back links,
     # so this isolates the streaming pattern from the porous geometry
     all_fluid = {}
@@ -655,9 +658,12 @@ def main():
                                    achieved_gbs=round(gbs, 1),
                                    hbm_frac=round(gbs / HBM_PEAK_GBS, 4),
                                    frac_of_sweep=round(gbs / sweep_gbs[sbytes], 4),
-                                   nonfinite=nonfinite)
+                                   nonfinite=nonfinite,
+                                   streaming="two-step" if streaming == abi.STREAM_TWO_STEP
+                                   else "aa")
             print(f"[bench] all-fluid periodic {n}^3 {name}: {mlups:.0f} MLUPS, {gbs:.0f} GB/s "
-                  f"({100 * gbs / HBM_PEAK_GBS:.1f}% HBM)", file=sys.stderr, flush=True)
+                  f"algorithmic ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)", file=sys.stderr,
+                  flush=True)
     head = results.get("f32") or next(iter(results.values()))
     head_name = "f32" if "f32" in results else next(iter(results))
     # the same workload through the drop-in time loop (lbf_run): residual
