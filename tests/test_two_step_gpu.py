@@ -58,6 +58,10 @@ def cases():
             force=1e-4 * frng.standard_normal((720, 3))),
         "moving_lid_d3q19_bgk": moving_lid,
         "moving_walls_d3q15_trt": walls_x,
+        "porous_mrt_d3q15": configs.porous_channel(
+            n=(11, 9, 10), velocity_set=abi.D3Q15,
+            op=Collision.mrt([1.0, 1.2, 1.3, 1.0, 1.1, 1.0, 1.1, 1.0, 1.1, 1.25, 1.25, 1.25,
+                              1.25, 1.25, 1.1]), force=1e-4, seed=17),
         "d3q27_bgk": configs.porous_channel(n=(10, 8, 9), velocity_set=abi.D3Q27,
                                             op=Collision.bgk(0.8), force=1e-4),
         "d3q27_mrt_sphere": configs.sphere_channel(
