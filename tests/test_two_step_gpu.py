@@ -44,6 +44,12 @@ def cases():
                           FaceBC.bounce_back(1.0, (0.03, 0.0, 0.0))],
                    node_type=geometry.bernoulli_mask((9, 12, 10), 0.15, 3),
                    initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0))
+    lid_d3q27 = Spec(velocity_set=abi.D3Q27, n=(10, 9, 8), collision=Collision.trt(1.1, 0.85),
+                     faces=[FaceBC.periodic(), FaceBC.periodic(), FaceBC.no_slip(),
+                            FaceBC.no_slip(), FaceBC.no_slip(),
+                            FaceBC.bounce_back(1.0, (0.04, 0.02, 0.0))],
+                     node_type=geometry.bernoulli_mask((10, 9, 8), 0.1, 6),
+                     initial_density=1.0, initial_velocity=(0.0, 0.0, 0.0))
     return {
         "porous_trt_force": configs.porous_channel(n=(24, 20, 16), force=1e-4),
         "porous_bgk_guo": configs.porous_channel(n=(16, 12, 20), op=Collision.bgk(0.7),
@@ -62,6 +68,7 @@ def cases():
             n=(11, 9, 10), velocity_set=abi.D3Q15,
             op=Collision.mrt([1.0, 1.2, 1.3, 1.0, 1.1, 1.0, 1.1, 1.0, 1.1, 1.25, 1.25, 1.25,
                               1.25, 1.25, 1.1]), force=1e-4, seed=17),
+        "moving_lid_d3q27_trt": lid_d3q27,
         "d3q27_bgk": configs.porous_channel(n=(10, 8, 9), velocity_set=abi.D3Q27,
                                             op=Collision.bgk(0.8), force=1e-4),
         "d3q27_mrt_sphere": configs.sphere_channel(
@@ -117,7 +124,8 @@ def test_f32_schemes_agree(name):
 
 
 @pytest.mark.parametrize("name", ["porous_trt_force", "moving_lid_d3q19_bgk", "d3q27_bgk",
-                                  "moving_walls_d3q15_trt"])
+                                  "moving_walls_d3q15_trt", "porous_mrt_d3q15",
+                                  "moving_lid_d3q27_trt"])
 def test_passes_f64_equal_oracle(name):
     sp = CASES[name].replace(precision=abi.F64)
     print(f"[two-step-oracle] {name}", flush=True)
